@@ -1,0 +1,192 @@
+"""Benchmark: config-2 graph on a 512^3 grid, fused/autotuned vs plain per-operator HIP.
+
+One step = one evaluation of
+    out = divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))
+plus sum/mean/variance/L2 of ``out`` on a 512^3 FP32 field f and a 512^3 x 3 channel-last
+vector field u (seeded Fourier inputs, periodic, dx = 1), inputs resident in HBM.
+
+Prints one JSON line.  ``value`` is the fused path's Gvoxel/s; ``roofline`` prices the fused
+kernel's algorithmic bytes (20 B/voxel: read f and u, write out) over its average duration
+measured with HIP events on the launch stream; ``cpu_baseline`` times the FP64 CPU oracle (the
+reference's torch eager semantics) on a bounded slab of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+C2_EXPR = "divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))"
+REDUCTIONS = ("sum", "mean", "variance", "norm_l2")
+HBM_PEAK_GBS = 8000.0
+
+
+def build_graph(f, u):
+    import stenpy_amd as sp
+    g, sink, _, _, _ = sp.compile_expression(C2_EXPR, dx=1.0, boundary="periodic",
+                                             field_map={"f": f, "u": u})
+    reds = [g.add(r, (sink,), {}) for r in REDUCTIONS]
+    return g, sink, reds
+
+
+def time_steps(fn, steps: int, warmup: int):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def kernel_ms(cg, args, nb, threads, reps: int) -> float:
+    """Average duration of the fused kernel alone, HIP events on the launch stream."""
+    import torch
+    from stenpy_amd import _lib as L
+    s = L.stream_ptr()
+    for _ in range(3):
+        cg.module.launch((nb, 1, 1), (threads, 1, 1), args, 0, s)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        cg.module.launch((nb, 1, 1), (threads, 1, 1), args, 0, s)
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / reps
+
+
+def fused_kernel_args(rt, g, res, sink, reds, out_dtype):
+    """Compiled group + packed args reproducing one evaluation's main kernel launch."""
+    import torch
+    from stenpy_amd import fused as F
+    ex = rt.fused
+    plan = next(iter(ex._plans.values())) if len(ex._plans) == 1 else None
+    key = [k for k, v in ex._plans.items()]
+    for k in key:
+        steps, info, comp = ex._plans[k]
+        for si, cg in comp.items():
+            if sink in cg.group.outputs and (out_dtype is None or cg.gen.out_dtypes.get(sink) == out_dtype):
+                srcs = [res[s] for s in cg.group.sources]
+                outs = [res[o] for o in cg.group.outputs]
+                tx, ty, zc, nb = cg.nblocks()
+                ws = [torch.empty(nb * 4, dtype=torch.float64, device=srcs[0].device) for _ in cg.group.reductions]
+                args = F.pack_params(cg, srcs, outs, ws, [], [], 0.0)
+                return cg, args, nb
+    raise RuntimeError("fused kernel not found")
+
+
+def cpu_baseline(n: int, planes: int):
+    """FP64 CPU oracle on a (planes, n, n) slab of the same graph; voxels/s."""
+    import torch
+    from stenpy_amd import gen
+    from oracle import graph_eval
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    shape = (planes, n, n)
+    f = torch.from_numpy(gen.generate_host(shape, 2)).double()
+    u = torch.stack([torch.from_numpy(gen.generate_host(shape, s)).double() for s in (3, 4, 5)], -1)
+    g, sink, reds = build_graph(f, u)
+    graph_eval.evaluate(g, torch.float64)           # warm-up
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        graph_eval.evaluate(g, torch.float64)
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    vox = planes * n * n
+    return vox / ts[1], threads, f"{planes}x{n}x{n} slab, FP64 torch CPU oracle, median of 3"
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-planes", type=int, default=32)
+    ap.add_argument("--traffic", type=str, default=None,
+                    help="JSON file with measured HBM bytes per launch of the fused kernel")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this backend runs on one GPU; --gpus must be 1"}))
+        return 2
+
+    import torch
+    import stenpy_amd as sp
+    from stenpy_amd import gen, tune
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    shape = (n, n, n)
+    f = gen.generate(shape, 2, dev)
+    u = gen.generate_vector(shape, (3, 4, 5), dev)
+    del_ = None
+    g, sink, reds = build_graph(f, u)
+    wanted = [sink] + reds
+    nvox = n ** 3
+
+    rt = sp.Runtime(device=dev, fusion_enabled=True, schedule_fn=tune.schedule_fn(dev))
+    res = {}
+
+    def fused_fp32():
+        res.update(rt.fused.run(g, rt, wanted=wanted))
+
+    def fused_bf16():
+        rt.fused.run(g, rt, wanted=wanted, out_dtype=torch.bfloat16)
+
+    def plain():
+        rt.run_eager(g, outputs=wanted)
+
+    t_fused = time_steps(fused_fp32, args.steps, args.warmup)
+    t_bf16 = time_steps(fused_bf16, args.steps, args.warmup)
+    t_plain = time_steps(plain, max(3, args.steps // 4), 1)
+
+    cg, kargs, nb = fused_kernel_args(rt, g, res, sink, reds, None)
+    k_ms = kernel_ms(cg, kargs, nb, cg.gen.sched.threads, max(10, args.steps))
+    alg_bytes = 20 * nvox
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+    traffic = None
+    if args.traffic and Path(args.traffic).exists():
+        traffic = json.loads(Path(args.traffic).read_text()).get("bytes_per_launch")
+
+    line = {
+        "metric": "Gvoxel/s + % HBM peak: fused/autotuned vs plain per-op HIP, 512^3 graph",
+        "value": round(nvox / t_fused / 1e9, 2),
+        "unit": "Gvoxel/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_fused * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded 8-mode Fourier fields + counter-based noise)",
+        "config": {"workload": f"config-2 graph, {n}^3 FP32 f + {n}^3x3 u, periodic, dx=1; "
+                               "out + sum/mean/variance/L2",
+                   "schedule": list(cg.gen.sched.as_tuple()),
+                   "hbm_frac_step": round(20 * nvox / t_fused / 1e9 / HBM_PEAK_GBS, 4),
+                   "plain_gvoxel_s": round(nvox / t_plain / 1e9, 3),
+                   "plain_ms_per_step": round(t_plain * 1e3, 3),
+                   "speedup_vs_plain": round(t_plain / t_fused, 2),
+                   "bf16_out_gvoxel_s": round(nvox / t_bf16 / 1e9, 2),
+                   "bf16_out_hbm_frac_step": round(18 * nvox / t_bf16 / 1e9 / HBM_PEAK_GBS, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel_ms": round(k_ms, 4),
+                     "algorithmic_bytes_per_launch": alg_bytes},
+    }
+    if not args.no_cpu:
+        rate, cores, sample = cpu_baseline(n, args.cpu_planes)
+        line["cpu_baseline"] = {"value": round(rate / 1e9, 5), "unit": "Gvoxel/s", "cores": cores,
+                                "kind": "port", "sample": sample}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,172 @@
+/*
+ * stenpy_hip.h -- C ABI of libstenpy_hip.so, the MI355X (gfx950) execution backend of Stenpy.
+ *
+ * The reference's plugin boundary is Python-level: ``register_operator(name, func, ...)``
+ * stores a callable that the eager executor invokes as ``func(*tensors, **params)``
+ * (stenpy_engine.py:1184-1212, called at 1074-1081).  This library is what those callables
+ * reach through ctypes: every entry point takes plain pointers, sizes and enums (no torch
+ * types), works on caller-owned device memory, is ordered on the caller's HIP stream, never
+ * throws or aborts across the ABI, and returns an ``int`` status (SP_OK = 0).  The message of
+ * the last failure on the calling thread is ``sp_last_error()``.
+ *
+ * Layout contract: fields are C-order (n0, n1, n2) with n2 contiguous; vector and tensor
+ * fields are channel-last, (n0, n1, n2, C) and (n0, n1, n2, 3, 3), exactly as the reference
+ * stores them (stenpy_engine.py:1861, 2127).  Storage is FP32 (BF16 allowed for outputs);
+ * reductions return FP64.
+ */
+#ifndef STENPY_HIP_H
+#define STENPY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SP_ABI_VERSION 3
+
+/* status codes */
+enum { SP_OK = 0, SP_EINVAL = 1, SP_EHIP = 2, SP_EUNSUPPORTED = 3, SP_ECOMPILE = 4 };
+
+/* element types */
+enum { SP_F32 = 0, SP_BF16 = 1, SP_F64 = 2 };
+
+/* boundary handling of padded stencils (stenpy_engine.py:1374-1391): neumann -> replicate,
+ * dirichlet -> zeros, periodic -> circular, reflect -> mirror; any other name pads zeros. */
+enum { SP_PERIODIC = 0, SP_NEUMANN = 1, SP_DIRICHLET = 2, SP_REFLECT = 3, SP_ZEROPAD = 4 };
+
+#define SP_MAX_DIMS 6
+
+/* A dense tensor view.  ``stride`` is in elements; a stride of 0 broadcasts. */
+typedef struct sp_tensor {
+    void*   ptr;
+    int32_t dtype;
+    int32_t ndim;
+    int64_t shape[SP_MAX_DIMS];
+    int64_t stride[SP_MAX_DIMS];
+} sp_tensor;
+
+/* ---- library -------------------------------------------------------------------------- */
+int         sp_abi_version(void);
+const char* sp_last_error(void);
+/* Name and compute-unit count of the current device. */
+int         sp_device_info(char* name, int name_len, int* n_cu, int* n_xcd);
+
+/* ---- point-wise (stenpy_engine.py:1218-1367) ------------------------------------------- */
+/* unary ops */
+enum { SP_NEG = 0, SP_EXP = 1, SP_LOG = 2, SP_SQRT = 3, SP_SIN = 4, SP_COS = 5, SP_TANH = 6,
+       SP_CLAMP = 7, SP_SCALE = 8, SP_COPY = 9, SP_SQUARE = 10 };
+/* binary ops */
+enum { SP_ADD = 0, SP_SUB = 1, SP_MUL = 2, SP_DIV = 3 };
+
+/* out = op(a; p0, p1).  LOG/SQRT clamp their argument below at p0 (eps), CLAMP uses
+ * [p0, p1], SCALE multiplies by p0.  ``out`` is contiguous, a is broadcast to it. */
+int sp_op_unary(int op, const sp_tensor* a, sp_tensor* out, double p0, double p1, void* stream);
+/* out = a op b with broadcasting; DIV replaces |b| < p0 by sign-preserving +-p0 (b = 0 -> +p0),
+ * stenpy_engine.py:1263-1279. */
+int sp_op_binary(int op, const sp_tensor* a, const sp_tensor* b, sp_tensor* out, double p0,
+                 void* stream);
+
+/* ---- channel-last helpers (stenpy_engine.py:2247-2313) --------------------------------- */
+/* out[..., :] = sum_c a[..., c] * b[..., c] folded left (extension op ``dot``). */
+int sp_op_dot(const sp_tensor* a, const sp_tensor* b, sp_tensor* out, void* stream);
+/* out[...] = sum_c a[..., c]^2 folded left (extension op ``sqnorm``). */
+int sp_op_sqnorm(const sp_tensor* a, sp_tensor* out, void* stream);
+/* out = max(|a|_2 over the last axis, eps), keepdim (``norm_last``). */
+int sp_op_norm_last(const sp_tensor* a, sp_tensor* out, double eps, void* stream);
+/* out[..., c] = a[..., c] / max(|a|_2, eps) (normalisation used by mean_curvature). */
+int sp_op_normalize_last(const sp_tensor* a, sp_tensor* out, double eps, void* stream);
+
+/* ---- stencils ---------------------------------------------------------------------------
+ * Axis pass: the building block for any rank.  The array is viewed as (outer, n, inner)
+ * along the differentiated axis; ``x`` may select one channel of a channel-last field via
+ * its strides.  Modes:
+ *   SP_D1_PERIODIC  (x[i+1] - x[i-1]) / (2h) with wrap           (torch.roll path, 1829-1834)
+ *   SP_D1_EDGE2     torch.gradient, 2nd-order one-sided edges      (torch.gradient edge_order=2)
+ *   SP_D1_EDGE1     torch.gradient, 1st-order one-sided edges      (edge_order=1)
+ *   SP_D2_PAD       (x[i+1] + x[i-1] - 2x) / h^2, ``pad`` = boundary enum (1501-1511)
+ * out = scale * value, or out += scale * value when ``accumulate``. */
+enum { SP_D1_PERIODIC = 0, SP_D1_EDGE2 = 1, SP_D1_EDGE1 = 2, SP_D2_PAD = 3 };
+int sp_axis_pass(const sp_tensor* x, int axis, sp_tensor* out, int mode, int pad, double h,
+                 double scale, int accumulate, void* stream);
+
+/* 3-D single-pass kernels for scalar (n0,n1,n2) / vector (n0,n1,n2,3) fields.
+ * Each replaces the CPU branch of the cited reference op (the parity target). */
+/* laplacian, stenpy_engine.py:1478-1512 */
+int sp_laplacian3d(const sp_tensor* x, sp_tensor* out, const double dx[3], int boundary,
+                   void* stream);
+/* gradient (dim=None), 1780-1865: periodic -> roll, else torch.gradient per axis */
+int sp_gradient3d(const sp_tensor* x, sp_tensor* out, double dx, int boundary, void* stream);
+/* divergence of (n0,n1,n2,3), 1870-1931 */
+int sp_divergence3d(const sp_tensor* v, sp_tensor* out, const double dx[3], int boundary,
+                    void* stream);
+/* curl of (n0,n1,n2,3), boundary ignored (torch.gradient edge_order=2), 1936-2032 */
+int sp_curl3d(const sp_tensor* v, sp_tensor* out, const double dx[3], void* stream);
+/* gradient_nd of a 3-D field without singleton axes, 2037-2081 */
+int sp_gradient_nd3d(const sp_tensor* x, sp_tensor* out, const double dx[3], void* stream);
+/* hessian H[i][j] = D_j D_i f with torch.gradient edges, 2086-2128 */
+int sp_hessian3d(const sp_tensor* x, sp_tensor* out, const double dx[3], void* stream);
+/* jacobian J[c][d] = D_d v_c, 2133-2175 (CPU branch) */
+int sp_jacobian3d(const sp_tensor* v, sp_tensor* out, const double dx[3], void* stream);
+/* mean curvature div(g/max(|g|,eps)), g = gradient_nd f, 2180-2195 */
+int sp_mean_curvature3d(const sp_tensor* x, sp_tensor* out, const double dx[3], int boundary,
+                        double eps, void* stream);
+/* material derivative sum_d v_d (gradient_nd f)_d, 2223-2243 */
+int sp_material_derivative3d(const sp_tensor* x, const sp_tensor* v, sp_tensor* out,
+                             const double dx[3], void* stream);
+
+/* ---- reductions (stenpy_engine.py:2340-2572, 2436-2457) -------------------------------
+ * Block partials are compensated FP32 (Neumaier sums, Welford/Chan for variance); one
+ * final block combines them in FP64, deterministically.  ``result`` is device memory:
+ *   SP_R_SUM, SP_R_MEAN, SP_R_NORM2, SP_R_VAR (unbiased), SP_R_VAR_BIASED: 1 double
+ *   SP_R_MINMAX: 2 doubles;  SP_R_INTEGRATE: 1 double (Simpson/trapezoid weights per axis,
+ *   ``wparams`` = {dx0, dx1, dx2, method}, method 0 = simpson, 1 = trapezoid; x is 3-D)
+ *   SP_R_ANYSMALLDIAG: 1 double = 1.0 if any |H[..., i, i]| < 1e-12 else 0.0 (x is (...,3,3))
+ * ``workspace`` must hold sp_reduce_workspace_bytes(numel) bytes of device memory. */
+enum { SP_R_SUM = 0, SP_R_MEAN = 1, SP_R_NORM2 = 2, SP_R_VAR = 3, SP_R_VAR_BIASED = 4,
+       SP_R_MINMAX = 5, SP_R_INTEGRATE = 6, SP_R_ANYSMALLDIAG = 7 };
+size_t sp_reduce_workspace_bytes(int64_t numel);
+/* Final FP64 combine of ``nrecords`` block records (4 doubles each: {s, c} for sums,
+ * {n, mean, M2} for variance, {lo, hi} for min/max) written by the fused kernels; ``count``
+ * is the number of reduced elements (used by mean and variance). */
+int sp_reduce_combine(int kind, const double* records, int nrecords, int64_t count,
+                      double* result, void* stream);
+int sp_reduce(int kind, const sp_tensor* x, double* result, void* workspace,
+              const double* wparams, void* stream);
+
+/* ---- per-voxel symmetric 3x3 tensor ops (stenpy_engine.py:2674-2730, 2852-2867) -------
+ *   SP_T_TRACE -> (...), SP_T_DET -> (...) [``flag`` = device double from SP_R_ANYSMALLDIAG:
+ *   when nonzero, det(H + 1e-12 I)], SP_T_EIG -> (..., 3) ascending (lower triangle, as
+ *   eigvalsh), SP_T_DEV -> (..., 3, 3), SP_T_INV -> (..., 3, 3). */
+enum { SP_T_TRACE = 0, SP_T_DET = 1, SP_T_EIG = 2, SP_T_DEV = 3, SP_T_INV = 4 };
+int sp_tensor3(int kind, const sp_tensor* h, sp_tensor* out, const double* flag, void* stream);
+
+/* ---- synthetic input ---------------------------------------------------------------------
+ * Planes [z0, z0 + out->shape[0]) of the seeded Fourier field of full shape n (FP64 formula,
+ * FP32 store); k: 8x3 wavenumbers, a: 8 amplitudes, phi: 8 phases (host arrays). */
+int sp_gen_fourier(sp_tensor* out, const int64_t n[3], int64_t z0, const int32_t* k,
+                   const double* a, const double* phi, uint64_t noise_seed, void* stream);
+
+/* ---- run-time compiled kernels (fused plans) ----------------------------------------------
+ * The fused executor generates CDNA4 HIP source per graph and compiles it with hipRTC. */
+typedef struct sp_module sp_module;
+/* Compile ``source`` for the current device; options are '\n'-separated.  On success
+ * *code/*code_size hold a malloc'ed code object (free with sp_free_code). */
+int  sp_rtc_compile(const char* source, const char* name, const char* options, void** code,
+                    size_t* code_size, char* log, size_t log_len);
+void sp_free_code(void* code);
+int  sp_module_load(const void* code, size_t code_size, const char* kernel_name,
+                    sp_module** mod);
+/* Launch the module's kernel with one by-value argument block of ``args_size`` bytes. */
+int  sp_module_launch(sp_module* mod, const unsigned grid[3], const unsigned block[3],
+                      unsigned shared_bytes, void* stream, const void* args, size_t args_size);
+/* registers, LDS bytes and spill bytes of the loaded kernel */
+int  sp_module_attributes(sp_module* mod, int* vgprs, int* lds_bytes, int* spill_bytes,
+                          int* max_threads);
+void sp_module_unload(sp_module* mod);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STENPY_HIP_H */

@@ -1,0 +1,289 @@
+// Global reductions (stenpy_engine.py:2340-2382 sum/mean/norm_l2/min_max, 2546-2572 variance,
+// 2415-2459 integrate, 2692-2696 the determinant's any(|diag| < 1e-12) test).
+//
+// Stage 1: a fixed grid of at most kMaxBlocks blocks; each thread walks a grid-stride range
+// and keeps compensated FP32 state (Neumaier sum, Welford mean/M2); lanes are merged with
+// cross-lane shuffles (two-sum for sums, Chan's formula for variance) and waves through
+// LDS; one FP64 record per block goes to the workspace.
+// Stage 2: one block merges the records in FP64 in a fixed order.  Both stages have a fixed
+// reduction order, so results are bit-reproducible (the reference's GPU variance uses float
+// atomics and is not).
+#include "sp_common.h"
+
+using namespace sp;
+
+namespace {
+
+constexpr int kMaxBlocks = 2048;
+constexpr int kThreads = 256;
+constexpr int kRec = 4;  // doubles per block record
+
+struct Acc {
+    float s, c;       // Neumaier sum of the primary quantity
+    float n, m, m2;   // Welford count / mean / M2
+    float lo, hi;     // min / max
+};
+
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ void welford_add(Acc& a, float x) {
+    a.n += 1.0f;
+    float d = x - a.m;
+    a.m += d / a.n;
+    a.m2 += d * (x - a.m);
+}
+
+__device__ __forceinline__ void merge(Acc& a, const Acc& b) {
+    float s, e;
+    two_sum(a.s, b.s, s, e);
+    a.s = s;
+    a.c = a.c + b.c + e;
+    float n = a.n + b.n;
+    if (b.n > 0.f) {
+        if (a.n == 0.f) {
+            a.m = b.m;
+            a.m2 = b.m2;
+        } else {
+            float d = b.m - a.m;
+            a.m = a.m + d * (b.n / n);
+            a.m2 = a.m2 + b.m2 + d * d * (a.n * b.n / n);
+        }
+    }
+    a.n = n;
+    a.lo = fminf(a.lo, b.lo);
+    a.hi = fmaxf(a.hi, b.hi);
+}
+
+__device__ __forceinline__ Acc shfl_acc(const Acc& a, int lane_mask) {
+    Acc b;
+    b.s = __shfl_xor(a.s, lane_mask);
+    b.c = __shfl_xor(a.c, lane_mask);
+    b.n = __shfl_xor(a.n, lane_mask);
+    b.m = __shfl_xor(a.m, lane_mask);
+    b.m2 = __shfl_xor(a.m2, lane_mask);
+    b.lo = __shfl_xor(a.lo, lane_mask);
+    b.hi = __shfl_xor(a.hi, lane_mask);
+    return b;
+}
+
+// Per-axis quadrature weight of index i on an axis of n points (reference Simpson/trapezoid).
+__device__ __forceinline__ double quad_w(int64_t i, int64_t n, double dx, int method) {
+    if (method == 1 || n < 3) {  // trapezoid (also Simpson's n < 3 branch)
+        if (n < 2) return 0.0;
+        return (i == 0 || i == n - 1) ? 0.5 * dx : dx;
+    }
+    const int64_t m = (n % 2 == 0) ? n - 1 : n;  // Simpson points
+    double w = 0.0;
+    if (i < m) {
+        if (i == 0 || i == m - 1) w = dx / 3.0;
+        else w = (i % 2 == 1) ? 4.0 * dx / 3.0 : 2.0 * dx / 3.0;
+    }
+    if (n % 2 == 0 && i >= n - 2) w += 0.5 * dx;  // trapezoid on the last interval
+    return w;
+}
+
+struct IntegrateGeom {
+    int64_t n0, n1, n2;
+    double dx0, dx1, dx2;
+    int method;
+};
+
+__global__ void __launch_bounds__(kThreads) k_reduce_stage1(int kind, const float* __restrict__ x,
+                                                            int64_t n, double* __restrict__ ws,
+                                                            IntegrateGeom ig) {
+    Acc a{0.f, 0.f, 0.f, 0.f, 0.f, INFINITY, -INFINITY};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (kind == SP_R_ANYSMALLDIAG) {
+        // x is (..., 3, 3); n counts matrices
+        for (int64_t v = t0; v < n; v += stride) {
+            const float* m = x + v * 9;
+            if (fabsf(m[0]) < 1e-12f || fabsf(m[4]) < 1e-12f || fabsf(m[8]) < 1e-12f) a.hi = 1.f;
+        }
+    } else if (kind == SP_R_INTEGRATE) {
+        for (int64_t i = t0; i < n; i += stride) {
+            const int64_t k = i % ig.n2;
+            const int64_t r = i / ig.n2;
+            const int64_t j = r % ig.n1;
+            const int64_t z = r / ig.n1;
+            const double w = quad_w(z, ig.n0, ig.dx0, ig.method) *
+                             quad_w(j, ig.n1, ig.dx1, ig.method) *
+                             quad_w(k, ig.n2, ig.dx2, ig.method);
+            const float term = x[i] * (float)w;
+            float s, e;
+            two_sum(a.s, term, s, e);
+            a.s = s;
+            a.c += e;
+        }
+    } else {
+        for (int64_t i = t0; i < n; i += stride) {
+            const float v = x[i];
+            if (kind == SP_R_SUM || kind == SP_R_MEAN) {
+                float s, e;
+                two_sum(a.s, v, s, e);
+                a.s = s;
+                a.c += e;
+            } else if (kind == SP_R_NORM2) {
+                float s, e;
+                two_sum(a.s, v * v, s, e);
+                a.s = s;
+                a.c += e;
+            } else if (kind == SP_R_VAR || kind == SP_R_VAR_BIASED) {
+                welford_add(a, v);
+            } else {  // min/max; NaN propagates like torch.min/max
+                if (v != v) { a.lo = v; a.hi = v; }
+                else if (a.lo == a.lo) { a.lo = fminf(a.lo, v); a.hi = fmaxf(a.hi, v); }
+            }
+        }
+    }
+    // wave reduction
+    for (int mask = 32; mask >= 1; mask >>= 1) {
+        Acc b = shfl_acc(a, mask);
+        if (kind == SP_R_MINMAX && (a.lo != a.lo || b.lo != b.lo)) {
+            a.lo = a.hi = NAN;
+        } else {
+            merge(a, b);
+        }
+    }
+    __shared__ Acc wave_acc[kThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wave_acc[wave] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Acc r = wave_acc[0];
+        for (int w = 1; w < kThreads / 64; ++w) {
+            if (kind == SP_R_MINMAX && (r.lo != r.lo || wave_acc[w].lo != wave_acc[w].lo))
+                r.lo = r.hi = NAN;
+            else
+                merge(r, wave_acc[w]);
+        }
+        double* rec = ws + (int64_t)blockIdx.x * kRec;
+        if (kind == SP_R_VAR || kind == SP_R_VAR_BIASED) {
+            rec[0] = r.n;
+            rec[1] = r.m;
+            rec[2] = r.m2;
+            rec[3] = 0.0;
+        } else if (kind == SP_R_MINMAX || kind == SP_R_ANYSMALLDIAG) {
+            rec[0] = r.lo;
+            rec[1] = r.hi;
+            rec[2] = rec[3] = 0.0;
+        } else {
+            rec[0] = r.s;
+            rec[1] = r.c;
+            rec[2] = rec[3] = 0.0;
+        }
+    }
+}
+
+struct Acc64 {
+    double s, c, n, m, m2, lo, hi;
+};
+
+__device__ __forceinline__ void merge64(Acc64& a, const Acc64& b, bool minmax) {
+    if (minmax) {
+        if (a.lo != a.lo || b.lo != b.lo) { a.lo = a.hi = NAN; return; }
+        a.lo = fmin(a.lo, b.lo);
+        a.hi = fmax(a.hi, b.hi);
+        return;
+    }
+    double t = a.s + b.s;
+    if (fabs(a.s) >= fabs(b.s)) a.c += (a.s - t) + b.s; else a.c += (b.s - t) + a.s;
+    a.s = t;
+    a.c += b.c;
+    double n = a.n + b.n;
+    if (b.n > 0.0) {
+        if (a.n == 0.0) { a.m = b.m; a.m2 = b.m2; }
+        else {
+            double d = b.m - a.m;
+            a.m = a.m + d * (b.n / n);
+            a.m2 = a.m2 + b.m2 + d * d * (a.n * b.n / n);
+        }
+    }
+    a.n = n;
+}
+
+__global__ void __launch_bounds__(kThreads) k_reduce_stage2(int kind, const double* __restrict__ ws,
+                                                            int nblocks, int64_t n,
+                                                            double* __restrict__ result) {
+    const bool mm = (kind == SP_R_MINMAX || kind == SP_R_ANYSMALLDIAG);
+    Acc64 a{0, 0, 0, 0, 0, INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nblocks; b += kThreads) {
+        const double* rec = ws + (int64_t)b * kRec;
+        Acc64 r{0, 0, 0, 0, 0, INFINITY, -INFINITY};
+        if (kind == SP_R_VAR || kind == SP_R_VAR_BIASED) { r.n = rec[0]; r.m = rec[1]; r.m2 = rec[2]; }
+        else if (mm) { r.lo = rec[0]; r.hi = rec[1]; }
+        else { r.s = rec[0]; r.c = rec[1]; }
+        merge64(a, r, mm);
+    }
+    __shared__ Acc64 part[kThreads];
+    part[threadIdx.x] = a;
+    __syncthreads();
+    for (int step = kThreads / 2; step >= 1; step >>= 1) {
+        if (threadIdx.x < step) merge64(part[threadIdx.x], part[threadIdx.x + step], mm);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const Acc64 r = part[0];
+        switch (kind) {
+            case SP_R_SUM: case SP_R_INTEGRATE: result[0] = r.s + r.c; break;
+            case SP_R_MEAN: result[0] = (r.s + r.c) / (double)n; break;
+            case SP_R_NORM2: result[0] = sqrt(r.s + r.c); break;
+            case SP_R_VAR: result[0] = r.m2 / (double)(n - 1); break;
+            case SP_R_VAR_BIASED: result[0] = r.m2 / (double)n; break;
+            case SP_R_MINMAX: result[0] = r.lo; result[1] = r.hi; break;
+            case SP_R_ANYSMALLDIAG: result[0] = r.hi > 0.0 ? 1.0 : 0.0; break;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" size_t sp_reduce_workspace_bytes(int64_t) {
+    return (size_t)kMaxBlocks * kRec * sizeof(double);
+}
+
+extern "C" int sp_reduce(int kind, const sp_tensor* x, double* result, void* workspace,
+                         const double* wparams, void* stream) {
+    SP_REQUIRE(x && x->dtype == SP_F32, "reduce: input must be f32");
+    SP_REQUIRE(is_contiguous(x), "reduce: input must be contiguous");
+    SP_REQUIRE(result && workspace, "reduce: result and workspace are required");
+    SP_REQUIRE(kind >= SP_R_SUM && kind <= SP_R_ANYSMALLDIAG, "reduce: unknown kind");
+    int64_t n = numel(x);
+    IntegrateGeom ig{1, 1, 1, 1.0, 1.0, 1.0, 0};
+    if (kind == SP_R_INTEGRATE) {
+        SP_REQUIRE(x->ndim == 3 && wparams, "integrate: expected a 3-D field and weights");
+        ig = IntegrateGeom{x->shape[0], x->shape[1], x->shape[2], wparams[0], wparams[1],
+                           wparams[2], (int)wparams[3]};
+    }
+    if (kind == SP_R_ANYSMALLDIAG) {
+        SP_REQUIRE(x->ndim >= 2 && x->shape[x->ndim - 1] == 3 && x->shape[x->ndim - 2] == 3,
+                   "any-small-diag: expected (..., 3, 3)");
+        n /= 9;
+    }
+    SP_REQUIRE(n > 0 || kind == SP_R_SUM || kind == SP_R_ANYSMALLDIAG || kind == SP_R_INTEGRATE,
+               "reduce: empty input");
+    int64_t blocks = (n + kThreads * 8 - 1) / (kThreads * 8);
+    if (blocks < 1) blocks = 1;
+    if (blocks > kMaxBlocks) blocks = kMaxBlocks;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_reduce_stage1, dim3((unsigned)blocks), dim3(kThreads), 0, st, kind,
+                       (const float*)x->ptr, n, (double*)workspace, ig);
+    int rc = launch_status("sp_reduce(stage1)");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(kThreads), 0, st, kind,
+                       (const double*)workspace, (int)blocks, n, result);
+    return launch_status("sp_reduce(stage2)");
+}
+
+extern "C" int sp_reduce_combine(int kind, const double* records, int nrecords, int64_t count,
+                                 double* result, void* stream) {
+    SP_REQUIRE(records && result && nrecords > 0, "reduce_combine: bad arguments");
+    SP_REQUIRE(kind >= SP_R_SUM && kind <= SP_R_ANYSMALLDIAG, "reduce_combine: unknown kind");
+    hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(kThreads), 0, as_stream(stream), kind,
+                       records, nrecords, count, result);
+    return launch_status("sp_reduce_combine");
+}

@@ -1,0 +1,203 @@
+"""Plain per-operator HIP kernels vs the CPU oracle (2x error rule, seeded Fourier inputs)."""
+import pytest
+import torch
+
+import stenpy_amd as sp
+from stenpy_amd import ops
+from oracle import ops as O
+from tests._util import assert_2x, field, vfield
+
+pytestmark = pytest.mark.gpu
+SHAPE = (20, 24, 28)
+
+
+def ref(fn, *args, **kw):
+    """(oracle64, ref32) of ``fn`` on the FP32 device inputs."""
+    a64 = [a.detach().cpu().double() if isinstance(a, torch.Tensor) else a for a in args]
+    a32 = [a.detach().cpu().float() if isinstance(a, torch.Tensor) else a for a in args]
+    return fn(*a64, **kw), fn(*a32, **kw)
+
+
+@pytest.mark.parametrize("op", ["neg", "exp", "log", "sqrt", "sin", "cos", "tanh"])
+def test_unary(op):
+    f = field(SHAPE, 11)
+    o64, o32 = ref(getattr(O, op), f)
+    assert_2x(getattr(ops, op)(f), o64, o32, op)
+
+
+def test_clamp_and_scalar_mul():
+    f = field(SHAPE, 12)
+    o64, o32 = ref(O.clamp, f, lo=-0.2, hi=0.3)
+    assert_2x(ops.clamp(f, lo=-0.2, hi=0.3), o64, o32, "clamp")
+    o64, o32 = ref(O.mul, f, scalar=1.7)
+    assert_2x(ops.mul(f, scalar=1.7), o64, o32, "mul scalar")
+
+
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "div"])
+def test_binary(op):
+    a, b = field(SHAPE, 13), field(SHAPE, 14)
+    b[0, 0, :4] = torch.tensor([0.0, -0.0, 1e-16, -1e-17], device=b.device)
+    o64, o32 = ref(getattr(O, op), a, b)
+    assert_2x(getattr(ops, op)(a, b), o64, o32, op)
+
+
+def test_binary_broadcast_scalar():
+    a = field(SHAPE, 15)
+    c = torch.tensor(2.5, dtype=torch.float64)
+    o64, o32 = ref(O.mul, a, c)
+    assert_2x(ops.mul(a, c), o64, o32, "mul 0-d")
+    o64, o32 = ref(O.div, c, a)
+    assert_2x(ops.div(c, a), o64, o32, "div 0-d")
+
+
+@pytest.mark.parametrize("bc", ["periodic", "neumann", "dirichlet", "reflect", "other"])
+@pytest.mark.parametrize("dx", [1.0, (0.5, 1.0, 2.0)])
+def test_laplacian(bc, dx):
+    f = field(SHAPE, 16)
+    o64, o32 = ref(O.laplacian, f, dx=dx, boundary=bc)
+    assert_2x(ops.laplacian(f, dx=dx, boundary=bc), o64, o32, f"laplacian {bc}")
+
+
+@pytest.mark.parametrize("bc", ["periodic", "neumann"])
+def test_laplacian_generic_rank(bc):
+    f2 = field((1, 30, 26), 17)[0].contiguous()
+    o64, o32 = ref(O.laplacian, f2, dx=0.5, boundary=bc)
+    assert_2x(ops.laplacian(f2, dx=0.5, boundary=bc), o64, o32, "laplacian 2-D")
+    v = vfield((10, 12, 14))
+    o64, o32 = ref(O.laplacian, v, dx=1.0, boundary=bc)
+    assert_2x(ops.laplacian(v, dx=1.0, boundary=bc), o64, o32, "laplacian 4-D quirk")
+
+
+@pytest.mark.parametrize("bc", ["periodic", "neumann"])
+@pytest.mark.parametrize("dim", [None, 0, 1, 2])
+def test_gradient(bc, dim):
+    f = field(SHAPE, 18)
+    o64, o32 = ref(O.gradient, f, dx=0.7, boundary=bc, dim=dim)
+    assert_2x(ops.gradient(f, dx=0.7, boundary=bc, dim=dim), o64, o32, f"gradient {bc} {dim}")
+
+
+def test_gradient_short_axis():
+    f = field((2, 9, 11), 19)
+    o64, o32 = ref(O.gradient, f, dx=1.0, boundary="neumann")
+    assert_2x(ops.gradient(f, dx=1.0, boundary="neumann"), o64, o32, "gradient n=2 axis")
+
+
+@pytest.mark.parametrize("bc", ["periodic", "neumann"])
+def test_divergence(bc):
+    v = vfield(SHAPE)
+    o64, o32 = ref(O.divergence, v, dx=(1.0, 0.5, 2.0), boundary=bc)
+    assert_2x(ops.divergence(v, dx=(1.0, 0.5, 2.0), boundary=bc), o64, o32, f"divergence {bc}")
+
+
+def test_curl_3d_and_2d():
+    v = vfield(SHAPE)
+    o64, o32 = ref(O.curl, v, dx=(0.5, 1.0, 1.5))
+    assert_2x(ops.curl(v, dx=(0.5, 1.0, 1.5)), o64, o32, "curl 3-D")
+    v2 = vfield((1, 18, 22), seeds=(6, 7))[0].contiguous()
+    o64, o32 = ref(O.curl, v2, dx=1.0)
+    assert_2x(ops.curl(v2, dx=1.0), o64, o32, "curl 2-D")
+
+
+def test_gradient_nd_hessian_jacobian():
+    f = field(SHAPE, 20)
+    o64, o32 = ref(O.gradient_nd, f, dx=(1.0, 0.5, 0.25))
+    assert_2x(ops.gradient_nd(f, dx=(1.0, 0.5, 0.25)), o64, o32, "gradient_nd")
+    o64, o32 = ref(O.hessian, f, dx=0.5)
+    assert_2x(ops.hessian(f, dx=0.5), o64, o32, "hessian")
+    v = vfield(SHAPE)
+    o64, o32 = ref(O.jacobian, v, dx=1.0)
+    assert_2x(ops.jacobian(v, dx=1.0), o64, o32, "jacobian")
+
+
+def test_gradient_nd_squeezed():
+    f = field((1, 16, 18), 21)
+    o64, o32 = ref(O.gradient_nd, f, dx=1.0)
+    assert_2x(ops.gradient_nd(f, dx=1.0), o64, o32, "gradient_nd squeeze")
+
+
+def test_hessian_quadratic_exact():
+    from stenpy_amd.gen import quadratic_host
+    q = torch.from_numpy(quadratic_host((12, 14, 16))).cuda()
+    h = ops.hessian(q, dx=1.0).cpu()
+    o64 = O.hessian(q.cpu().double(), dx=1.0)
+    assert torch.equal(h.double(), o64), "integer quadratic hessian must be exact"
+
+
+@pytest.mark.parametrize("bc", ["periodic", "neumann"])
+def test_mean_curvature(bc):
+    f = field(SHAPE, 22)
+    o64, o32 = ref(O.mean_curvature, f, dx=1.0, boundary=bc)
+    assert_2x(ops.mean_curvature(f, dx=1.0, boundary=bc), o64, o32, f"mean_curvature {bc}")
+
+
+def test_material_derivative_surface_normals():
+    f, v = field(SHAPE, 23), vfield(SHAPE)
+    o64, o32 = ref(O.material_derivative, f, v, dx=1.0)
+    assert_2x(ops.material_derivative(f, v, dx=1.0), o64, o32, "material_derivative")
+    h = field((1, 20, 24), 24)[0].contiguous()
+    o64, o32 = ref(O.surface_normals, h, dx=1.0)
+    assert_2x(ops.surface_normals(h, dx=1.0), o64, o32, "surface_normals")
+
+
+def test_channel_helpers():
+    u, g = vfield(SHAPE), vfield(SHAPE, seeds=(8, 9, 10))
+    o64, o32 = ref(O.dot, u, g)
+    assert_2x(ops.dot(u, g), o64, o32, "dot")
+    o64, o32 = ref(O.sqnorm, u)
+    assert_2x(ops.sqnorm(u), o64, o32, "sqnorm")
+    o64, o32 = ref(O.norm_last, u)
+    assert_2x(ops.norm_last(u), o64, o32, "norm_last")
+    o64, o32 = ref(O.select_component, u, 1)
+    assert_2x(ops.select_component(u, 1), o64, o32, "select_component")
+
+
+@pytest.mark.parametrize("kind", ["sum", "mean", "norm_l2", "variance", "min_max"])
+def test_reductions(kind):
+    f = field((32, 33, 34), 25) + 0.5
+    fn = {"sum": O.op_sum, "mean": O.mean, "norm_l2": O.norm_l2, "variance": O.variance,
+          "min_max": O.min_max}[kind]
+    o64, o32 = ref(fn, f)
+    b = getattr(ops, {"sum": "op_sum"}.get(kind, kind))(f)
+    assert b.dtype == torch.float64
+    assert_2x(b, o64, o32, kind)
+
+
+@pytest.mark.parametrize("method", ["simpson", "trapezoid"])
+@pytest.mark.parametrize("shape", [(20, 24, 28), (21, 25, 2)])
+def test_integrate(method, shape):
+    f = field(shape, 26)
+    o64, o32 = ref(O.integrate, f, dx=(0.5, 1.0, 0.25), method=method)
+    assert_2x(ops.integrate(f, dx=(0.5, 1.0, 0.25), method=method), o64, o32, "integrate")
+
+
+def test_tensor_ops():
+    f = field(SHAPE, 27)
+    h = ops.hessian(f, dx=1.0)
+    for name in ["trace", "determinant", "eigenvalues", "deviatoric", "inverse"]:
+        o64, o32 = ref(getattr(O, name), h)
+        assert_2x(getattr(ops, name)(h), o64, o32, name)
+
+
+def test_determinant_global_regularisation():
+    f = field((10, 10, 10), 28)
+    h = ops.hessian(f, dx=1.0)
+    h[3, 3, 3, 1, 1] = 0.0                      # one tiny diagonal entry switches on +1e-12 I
+    o64, o32 = ref(O.determinant, h)
+    assert_2x(ops.determinant(h), o64, o32, "det regularised")
+
+
+def test_eigenvalues_degenerate_branches():
+    from stenpy_amd.gen import quadratic_host
+    q = torch.from_numpy(quadratic_host((8, 9, 10))).cuda()
+    e = ops.eigenvalues(ops.hessian(q, dx=1.0)).cpu()
+    assert torch.equal(e, torch.full_like(e, 2.0)), "2I must give exactly (2, 2, 2)"
+    # near-double-degenerate matrices take the FP64 branch
+    g = torch.Generator().manual_seed(0)
+    Q, _ = torch.linalg.qr(torch.randn(4096, 3, 3, generator=g, dtype=torch.float64))
+    lam = torch.stack([torch.ones(4096, dtype=torch.float64),
+                       1 + 1e-4 * torch.randn(4096, generator=g, dtype=torch.float64),
+                       3 * torch.randn(4096, generator=g, dtype=torch.float64)], -1)
+    A = (Q @ torch.diag_embed(lam) @ Q.transpose(1, 2)).float()
+    A = 0.5 * (A + A.transpose(1, 2))
+    o64, o32 = ref(O.eigenvalues, A)
+    assert_2x(ops.eigenvalues(A.cuda()), o64, o32, "eig near-degenerate")
