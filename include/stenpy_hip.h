@@ -117,7 +117,7 @@ int sp_material_derivative3d(const sp_tensor* x, const sp_tensor* v, sp_tensor* 
                              const double dx[3], void* stream);
 
 /* ---- reductions (stenpy_engine.py:2340-2572, 2436-2457) -------------------------------
- * Block partials are compensated FP32 (Neumaier sums, Welford/Chan for variance); one
+ * Block partials are compensated FP32 sums (x, and exactly split x^2 for variance); one
  * final block combines them in FP64, deterministically.  ``result`` is device memory:
  *   SP_R_SUM, SP_R_MEAN, SP_R_NORM2, SP_R_VAR (unbiased), SP_R_VAR_BIASED: 1 double
  *   SP_R_MINMAX: 2 doubles;  SP_R_INTEGRATE: 1 double (Simpson/trapezoid weights per axis,
@@ -128,7 +128,7 @@ enum { SP_R_SUM = 0, SP_R_MEAN = 1, SP_R_NORM2 = 2, SP_R_VAR = 3, SP_R_VAR_BIASE
        SP_R_MINMAX = 5, SP_R_INTEGRATE = 6, SP_R_ANYSMALLDIAG = 7 };
 size_t sp_reduce_workspace_bytes(int64_t numel);
 /* Final FP64 combine of ``nrecords`` block records (4 doubles each: {s, c} for sums,
- * {n, mean, M2} for variance, {lo, hi} for min/max) written by the fused kernels; ``count``
+ * {s1, c1, s2, c2} for variance, {lo, hi} for min/max) written by the fused kernels; ``count``
  * is the number of reduced elements (used by mean and variance). */
 int sp_reduce_combine(int kind, const double* records, int nrecords, int64_t count,
                       double* result, void* stream);
@@ -165,6 +165,29 @@ int  sp_module_launch(sp_module* mod, const unsigned grid[3], const unsigned blo
 int  sp_module_attributes(sp_module* mod, int* vgprs, int* lds_bytes, int* spill_bytes,
                           int* max_threads);
 void sp_module_unload(sp_module* mod);
+
+/* ---- schedule model (MLP trained on autotuner timings) --------------------------------
+ * The one dense contraction of the backend, on MFMA: C[M,N] (+)= alpha * op(A) op(B) with
+ * bf16 row-major A (M x K, or K x M when ta) and B (K x N, or N x K when tb), fp32 C.
+ * No counterpart in the reference (its cost model is static, fused_compiler.py:20-43). */
+int sp_gemm_bf16(int ta, int tb, int M, int N, int K, const void* A, int lda, const void* B,
+                 int ldb, float* C, int ldc, float alpha, int accumulate, void* stream);
+/* z = x + bias[col] (fp32, optional) and a = relu(z) or z as bf16 (optional) */
+int sp_mlp_bias_act(const float* x, const float* bias, int rows, int cols, int relu, float* z,
+                    void* a_bf16, void* stream);
+/* ReLU backward: g_out = g * (z > 0) as fp32 and/or bf16 */
+int sp_mlp_relu_grad(const float* g, const float* z, int64_t n, float* g_out, void* g_out_bf16,
+                     void* stream);
+/* out[c] = sum_r x[r, c] (compensated, fixed order) */
+int sp_mlp_colsum(const float* x, int rows, int cols, float* out, void* stream);
+/* loss = mean((y - t)^2) (device scalar), grad = 2 (y - t) / n (fp32 and/or bf16) */
+int sp_mlp_mse(const float* y, const float* t, int n, float* grad, void* grad_bf16, float* loss,
+               void* stream);
+/* Adam step (bias-corrected, decoupled-free weight decay added to the gradient) on fp32
+ * master weights; refreshes the bf16 copy when given */
+int sp_mlp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
+                float beta2, float eps, int step, float weight_decay, void* w_bf16, void* stream);
+int sp_cast_f32_bf16(const float* x, int64_t n, void* y, void* stream);
 
 #ifdef __cplusplus
 }

@@ -85,6 +85,7 @@ def _declare(lib: C.CDLL) -> None:
         "sp_material_derivative3d": ([P, P, P, C.POINTER(D), V], I),
         "sp_reduce_workspace_bytes": ([C.c_int64], Z),
         "sp_reduce": ([I, P, V, V, C.POINTER(D), V], I),
+        "sp_reduce_combine": ([I, V, I, C.c_int64, V, V], I),
         "sp_tensor3": ([I, P, P, V, V], I),
         "sp_gen_fourier": ([P, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int32),
                             C.POINTER(D), C.POINTER(D), C.c_uint64, V], I),
@@ -95,6 +96,14 @@ def _declare(lib: C.CDLL) -> None:
         "sp_module_launch": ([V, C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.c_uint, V, V, Z], I),
         "sp_module_attributes": ([V, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)], I),
         "sp_module_unload": ([V], None),
+        "sp_gemm_bf16": ([I, I, I, I, I, V, I, V, I, V, I, C.c_float, I, V], I),
+        "sp_mlp_bias_act": ([V, V, I, I, I, V, V, V], I),
+        "sp_mlp_relu_grad": ([V, V, C.c_int64, V, V, V], I),
+        "sp_mlp_colsum": ([V, I, I, V, V], I),
+        "sp_mlp_mse": ([V, V, I, V, V, V, V], I),
+        "sp_mlp_adam": ([V, V, V, V, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, I,
+                         C.c_float, V, V], I),
+        "sp_cast_f32_bf16": ([V, C.c_int64, V, V], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -4,7 +4,7 @@
 // self-contained.  The generated part defines the per-graph evaluation functions; this file
 // provides coordinate transforms, LDS index mapping, BF16 stores, compensated block
 // reductions and the reduction record layout shared with sp_reduce.hip's final combine
-// (4 doubles per block: {s, c} sums, {n, mean, M2} variance, {lo, hi} min/max).
+// (4 doubles per block: {s, c} sums, {s1, c1, s2, c2} variance, {lo, hi} min/max).
 #pragma once
 
 #ifndef INFINITY
@@ -62,16 +62,21 @@ struct RSum {  // Neumaier sum
         s = t;
     }
 };
-struct RVar {  // Welford
-    float n, m, m2;
-    __device__ __forceinline__ void init() { n = 0.f; m = 0.f; m2 = 0.f; }
+struct RVar {  // compensated first and second moments (exact squares via FMA)
+    RSum s1, s2;
+    __device__ __forceinline__ void init() { s1.init(); s2.init(); }
     __device__ __forceinline__ void add(float x) {
-        n += 1.f;
-        float d = x - m;
-        m += d / n;
-        m2 += d * (x - m);
+        s1.add(x);
+        const float q = x * x;
+        s2.add(q);
+        s2.c += fmaf(x, x, -q);
     }
 };
+__device__ __forceinline__ void rsum_add_sq(RSum& a, float x) {  // a += x*x exactly split
+    const float q = x * x;
+    a.add(q);
+    a.c += fmaf(x, x, -q);
+}
 struct RMinMax {
     float lo, hi;
     __device__ __forceinline__ void init() { lo = INFINITY; hi = -INFINITY; }
@@ -88,22 +93,13 @@ __device__ __forceinline__ void merge_sum(RSum& a, float s, float c) {
     a.s = t;
     a.c = a.c + c + e;
 }
-__device__ __forceinline__ void merge_var(RVar& a, float n, float m, float m2) {
-    if (n == 0.f) return;
-    if (a.n == 0.f) { a.n = n; a.m = m; a.m2 = m2; return; }
-    float t = a.n + n;
-    float d = m - a.m;
-    a.m = a.m + d * (n / t);
-    a.m2 = a.m2 + m2 + d * d * (a.n * n / t);
-    a.n = t;
-}
 __device__ __forceinline__ void merge_mm(RMinMax& a, float lo, float hi) {
     if (a.lo != a.lo || lo != lo) { a.lo = a.hi = NAN; return; }
     a.lo = fminf(a.lo, lo);
     a.hi = fmaxf(a.hi, hi);
 }
 
-// Block-wide reductions; ``scratch`` holds at least blockDim.x/64 * 3 floats.  Thread 0 of
+// Block-wide reductions; ``scratch`` holds at least blockDim.x/64 * 4 floats.  Thread 0 of
 // the block writes the 4-double record.
 __device__ __forceinline__ void block_sum(RSum a, float* scratch, double* rec) {
     for (int m = 32; m >= 1; m >>= 1) merge_sum(a, __shfl_xor(a.s, m), __shfl_xor(a.c, m));
@@ -118,17 +114,25 @@ __device__ __forceinline__ void block_sum(RSum a, float* scratch, double* rec) {
     }
 }
 __device__ __forceinline__ void block_var(RVar a, float* scratch, double* rec) {
-    for (int m = 32; m >= 1; m >>= 1)
-        merge_var(a, __shfl_xor(a.n, m), __shfl_xor(a.m, m), __shfl_xor(a.m2, m));
+    for (int m = 32; m >= 1; m >>= 1) {
+        merge_sum(a.s1, __shfl_xor(a.s1.s, m), __shfl_xor(a.s1.c, m));
+        merge_sum(a.s2, __shfl_xor(a.s2.s, m), __shfl_xor(a.s2.c, m));
+    }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     __syncthreads();
-    if (lane == 0) { scratch[3 * w] = a.n; scratch[3 * w + 1] = a.m; scratch[3 * w + 2] = a.m2; }
+    if (lane == 0) {
+        scratch[4 * w] = a.s1.s; scratch[4 * w + 1] = a.s1.c;
+        scratch[4 * w + 2] = a.s2.s; scratch[4 * w + 3] = a.s2.c;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        RVar r; r.n = scratch[0]; r.m = scratch[1]; r.m2 = scratch[2];
-        for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
-            merge_var(r, scratch[3 * i], scratch[3 * i + 1], scratch[3 * i + 2]);
-        rec[0] = r.n; rec[1] = r.m; rec[2] = r.m2; rec[3] = 0.0;
+        RSum r1, r2;
+        r1.s = scratch[0]; r1.c = scratch[1]; r2.s = scratch[2]; r2.c = scratch[3];
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+            merge_sum(r1, scratch[4 * i], scratch[4 * i + 1]);
+            merge_sum(r2, scratch[4 * i + 2], scratch[4 * i + 3]);
+        }
+        rec[0] = r1.s; rec[1] = r1.c; rec[2] = r2.s; rec[3] = r2.c;
     }
 }
 __device__ __forceinline__ void block_mm(RMinMax a, float* scratch, double* rec) {

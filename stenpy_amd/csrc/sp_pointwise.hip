@@ -20,12 +20,14 @@ struct Bcast {
 __device__ __forceinline__ float unary_f(int op, float a, float p0, float p1) {
     switch (op) {
         case SP_NEG: return -a;
-        case SP_EXP: return expf(a);
-        case SP_LOG: return logf(a < p0 ? p0 : a);
+        // transcendentals: FP64 evaluation rounded once to FP32 (the FP32 library versions
+        // are 1-2 ulp; the reference's CPU path is near correctly rounded)
+        case SP_EXP: return (float)exp((double)a);
+        case SP_LOG: return (float)log((double)(a < p0 ? p0 : a));
         case SP_SQRT: return sqrtf(a < p0 ? p0 : a);
-        case SP_SIN: return sinf(a);
-        case SP_COS: return cosf(a);
-        case SP_TANH: return tanhf(a);
+        case SP_SIN: return (float)sin((double)a);
+        case SP_COS: return (float)cos((double)a);
+        case SP_TANH: return (float)tanh((double)a);
         case SP_CLAMP: return a < p0 ? p0 : (a > p1 ? p1 : a);
         case SP_SCALE: return a * p0;
         case SP_SQUARE: return a * a;

@@ -2,8 +2,9 @@
 // 2415-2459 integrate, 2692-2696 the determinant's any(|diag| < 1e-12) test).
 //
 // Stage 1: a fixed grid of at most kMaxBlocks blocks; each thread walks a grid-stride range
-// and keeps compensated FP32 state (Neumaier sum, Welford mean/M2); lanes are merged with
-// cross-lane shuffles (two-sum for sums, Chan's formula for variance) and waves through
+// and keeps compensated FP32 sums (two-sum error terms; squares split exactly with an FMA,
+// so variance = (S2 - S1^2/n)/(n-1) is formed in FP64 at the end); lanes are merged with
+// cross-lane shuffles and waves through
 // LDS; one FP64 record per block goes to the workspace.
 // Stage 2: one block merges the records in FP64 in a fixed order.  Both stages have a fixed
 // reduction order, so results are bit-reproducible (the reference's GPU variance uses float
@@ -19,8 +20,8 @@ constexpr int kThreads = 256;
 constexpr int kRec = 4;  // doubles per block record
 
 struct Acc {
-    float s, c;       // Neumaier sum of the primary quantity
-    float n, m, m2;   // Welford count / mean / M2
+    float s, c;       // compensated sum of the primary quantity (x, x^2 or w x)
+    float s2, c2;     // compensated sum of x^2 (variance)
     float lo, hi;     // min / max
 };
 
@@ -30,11 +31,17 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
     e = (a - (s - bb)) + (b - bb);
 }
 
-__device__ __forceinline__ void welford_add(Acc& a, float x) {
-    a.n += 1.0f;
-    float d = x - a.m;
-    a.m += d / a.n;
-    a.m2 += d * (x - a.m);
+// a.s/a.c += x and a.s2/a.c2 += x^2, the square split exactly with an FMA
+__device__ __forceinline__ void add_sum(float& s, float& c, float x) {
+    float t, e;
+    two_sum(s, x, t, e);
+    s = t;
+    c += e;
+}
+__device__ __forceinline__ void add_sq(float& s, float& c, float x) {
+    const float q = x * x;
+    add_sum(s, c, q);
+    c += fmaf(x, x, -q);
 }
 
 __device__ __forceinline__ void merge(Acc& a, const Acc& b) {
@@ -42,18 +49,9 @@ __device__ __forceinline__ void merge(Acc& a, const Acc& b) {
     two_sum(a.s, b.s, s, e);
     a.s = s;
     a.c = a.c + b.c + e;
-    float n = a.n + b.n;
-    if (b.n > 0.f) {
-        if (a.n == 0.f) {
-            a.m = b.m;
-            a.m2 = b.m2;
-        } else {
-            float d = b.m - a.m;
-            a.m = a.m + d * (b.n / n);
-            a.m2 = a.m2 + b.m2 + d * d * (a.n * b.n / n);
-        }
-    }
-    a.n = n;
+    two_sum(a.s2, b.s2, s, e);
+    a.s2 = s;
+    a.c2 = a.c2 + b.c2 + e;
     a.lo = fminf(a.lo, b.lo);
     a.hi = fmaxf(a.hi, b.hi);
 }
@@ -62,9 +60,8 @@ __device__ __forceinline__ Acc shfl_acc(const Acc& a, int lane_mask) {
     Acc b;
     b.s = __shfl_xor(a.s, lane_mask);
     b.c = __shfl_xor(a.c, lane_mask);
-    b.n = __shfl_xor(a.n, lane_mask);
-    b.m = __shfl_xor(a.m, lane_mask);
-    b.m2 = __shfl_xor(a.m2, lane_mask);
+    b.s2 = __shfl_xor(a.s2, lane_mask);
+    b.c2 = __shfl_xor(a.c2, lane_mask);
     b.lo = __shfl_xor(a.lo, lane_mask);
     b.hi = __shfl_xor(a.hi, lane_mask);
     return b;
@@ -95,7 +92,7 @@ struct IntegrateGeom {
 __global__ void __launch_bounds__(kThreads) k_reduce_stage1(int kind, const float* __restrict__ x,
                                                             int64_t n, double* __restrict__ ws,
                                                             IntegrateGeom ig) {
-    Acc a{0.f, 0.f, 0.f, 0.f, 0.f, INFINITY, -INFINITY};
+    Acc a{0.f, 0.f, 0.f, 0.f, INFINITY, -INFINITY};
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (kind == SP_R_ANYSMALLDIAG) {
@@ -113,27 +110,18 @@ __global__ void __launch_bounds__(kThreads) k_reduce_stage1(int kind, const floa
             const double w = quad_w(z, ig.n0, ig.dx0, ig.method) *
                              quad_w(j, ig.n1, ig.dx1, ig.method) *
                              quad_w(k, ig.n2, ig.dx2, ig.method);
-            const float term = x[i] * (float)w;
-            float s, e;
-            two_sum(a.s, term, s, e);
-            a.s = s;
-            a.c += e;
+            add_sum(a.s, a.c, x[i] * (float)w);
         }
     } else {
         for (int64_t i = t0; i < n; i += stride) {
             const float v = x[i];
             if (kind == SP_R_SUM || kind == SP_R_MEAN) {
-                float s, e;
-                two_sum(a.s, v, s, e);
-                a.s = s;
-                a.c += e;
+                add_sum(a.s, a.c, v);
             } else if (kind == SP_R_NORM2) {
-                float s, e;
-                two_sum(a.s, v * v, s, e);
-                a.s = s;
-                a.c += e;
+                add_sq(a.s, a.c, v);
             } else if (kind == SP_R_VAR || kind == SP_R_VAR_BIASED) {
-                welford_add(a, v);
+                add_sum(a.s, a.c, v);
+                add_sq(a.s2, a.c2, v);
             } else {  // min/max; NaN propagates like torch.min/max
                 if (v != v) { a.lo = v; a.hi = v; }
                 else if (a.lo == a.lo) { a.lo = fminf(a.lo, v); a.hi = fmaxf(a.hi, v); }
@@ -163,10 +151,10 @@ __global__ void __launch_bounds__(kThreads) k_reduce_stage1(int kind, const floa
         }
         double* rec = ws + (int64_t)blockIdx.x * kRec;
         if (kind == SP_R_VAR || kind == SP_R_VAR_BIASED) {
-            rec[0] = r.n;
-            rec[1] = r.m;
-            rec[2] = r.m2;
-            rec[3] = 0.0;
+            rec[0] = r.s;
+            rec[1] = r.c;
+            rec[2] = r.s2;
+            rec[3] = r.c2;
         } else if (kind == SP_R_MINMAX || kind == SP_R_ANYSMALLDIAG) {
             rec[0] = r.lo;
             rec[1] = r.hi;
@@ -180,8 +168,14 @@ __global__ void __launch_bounds__(kThreads) k_reduce_stage1(int kind, const floa
 }
 
 struct Acc64 {
-    double s, c, n, m, m2, lo, hi;
+    double s, c, s2, c2, lo, hi;
 };
+
+__device__ __forceinline__ void nsum64(double& s, double& c, double x) {
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
+    s = t;
+}
 
 __device__ __forceinline__ void merge64(Acc64& a, const Acc64& b, bool minmax) {
     if (minmax) {
@@ -190,31 +184,21 @@ __device__ __forceinline__ void merge64(Acc64& a, const Acc64& b, bool minmax) {
         a.hi = fmax(a.hi, b.hi);
         return;
     }
-    double t = a.s + b.s;
-    if (fabs(a.s) >= fabs(b.s)) a.c += (a.s - t) + b.s; else a.c += (b.s - t) + a.s;
-    a.s = t;
+    nsum64(a.s, a.c, b.s);
     a.c += b.c;
-    double n = a.n + b.n;
-    if (b.n > 0.0) {
-        if (a.n == 0.0) { a.m = b.m; a.m2 = b.m2; }
-        else {
-            double d = b.m - a.m;
-            a.m = a.m + d * (b.n / n);
-            a.m2 = a.m2 + b.m2 + d * d * (a.n * b.n / n);
-        }
-    }
-    a.n = n;
+    nsum64(a.s2, a.c2, b.s2);
+    a.c2 += b.c2;
 }
 
 __global__ void __launch_bounds__(kThreads) k_reduce_stage2(int kind, const double* __restrict__ ws,
                                                             int nblocks, int64_t n,
                                                             double* __restrict__ result) {
     const bool mm = (kind == SP_R_MINMAX || kind == SP_R_ANYSMALLDIAG);
-    Acc64 a{0, 0, 0, 0, 0, INFINITY, -INFINITY};
+    Acc64 a{0, 0, 0, 0, INFINITY, -INFINITY};
     for (int b = threadIdx.x; b < nblocks; b += kThreads) {
         const double* rec = ws + (int64_t)b * kRec;
-        Acc64 r{0, 0, 0, 0, 0, INFINITY, -INFINITY};
-        if (kind == SP_R_VAR || kind == SP_R_VAR_BIASED) { r.n = rec[0]; r.m = rec[1]; r.m2 = rec[2]; }
+        Acc64 r{0, 0, 0, 0, INFINITY, -INFINITY};
+        if (kind == SP_R_VAR || kind == SP_R_VAR_BIASED) { r.s = rec[0]; r.c = rec[1]; r.s2 = rec[2]; r.c2 = rec[3]; }
         else if (mm) { r.lo = rec[0]; r.hi = rec[1]; }
         else { r.s = rec[0]; r.c = rec[1]; }
         merge64(a, r, mm);
@@ -232,8 +216,13 @@ __global__ void __launch_bounds__(kThreads) k_reduce_stage2(int kind, const doub
             case SP_R_SUM: case SP_R_INTEGRATE: result[0] = r.s + r.c; break;
             case SP_R_MEAN: result[0] = (r.s + r.c) / (double)n; break;
             case SP_R_NORM2: result[0] = sqrt(r.s + r.c); break;
-            case SP_R_VAR: result[0] = r.m2 / (double)(n - 1); break;
-            case SP_R_VAR_BIASED: result[0] = r.m2 / (double)n; break;
+            case SP_R_VAR:
+            case SP_R_VAR_BIASED: {
+                const double s1 = r.s + r.c, s2 = r.s2 + r.c2;
+                const double m2 = s2 - s1 * (s1 / (double)n);
+                result[0] = m2 / (double)(kind == SP_R_VAR ? n - 1 : n);
+                break;
+            }
             case SP_R_MINMAX: result[0] = r.lo; result[1] = r.hi; break;
             case SP_R_ANYSMALLDIAG: result[0] = r.hi > 0.0 ? 1.0 : 0.0; break;
         }

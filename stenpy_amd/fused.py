@@ -516,16 +516,18 @@ class _Emit:
     def _unary(op: str, x: str, p: Dict) -> str:
         if op == "neg":
             return f"-{x}"
+        # transcendentals are evaluated in FP64 and rounded once (FP32 library versions are
+        # 1-2 ulp, the reference CPU path is near correctly rounded)
         if op == "exp":
-            return f"expf({x})"
+            return f"(float)exp((double){x})"
         if op == "log":
             e = _flt(p.get("eps", 1e-15))
-            return f"logf({x} < {e} ? {e} : {x})"
+            return f"(float)log((double)({x} < {e} ? {e} : {x}))"
         if op == "sqrt":
             e = _flt(p.get("eps", 0.0))
             return f"sqrtf({x} < {e} ? {e} : {x})"
         if op in ("sin", "cos", "tanh"):
-            return f"{op}f({x})"
+            return f"(float){op}((double){x})"
         if op == "clamp":
             lo, hi = _flt(p.get("lo", 0.0)), _flt(p.get("hi", 1.0))
             return f"({x} < {lo} ? {lo} : ({x} > {hi} ? {hi} : {x}))"
@@ -810,7 +812,7 @@ class KernelGen:
                 body.append(f"A.r{ri}.add({vals[0]} * ({w}));")
             elif node.op_name == "norm_l2":
                 for v in vals:
-                    body.append(f"A.r{ri}.add({v} * {v});")
+                    body.append(f"rsum_add_sq(A.r{ri}, {v});")
             else:
                 for v in vals:
                     body.append(f"A.r{ri}.add({v});")
@@ -892,7 +894,7 @@ class KernelGen:
         ns = len(g.sources)
         out = []
         out.append("extern \"C\" __global__ void __launch_bounds__(NT) fused_kernel(const Params p) {")
-        out.append("    __shared__ float red_scratch[3 * NT / 64 + 3];")
+        out.append("    __shared__ float red_scratch[4 * NT / 64 + 4];")
         out.append("    const int tid = threadIdx.x;")
         # XCD-aware block remap: XCD k (hardware block id % 8) walks a contiguous tile range.
         out.append("    const int b = blockIdx.x, nb = p.nblocks, per = nb >> 3, rem = nb & 7;")
@@ -910,6 +912,9 @@ class KernelGen:
             out.append(f"    A.sc{i} = (float)p.scal{i}[0];")
         out.append(f"    const bool ex = (x0 < REDGE) || (x0 + TX - 1 + REDGE > N2 - 1);")
         out.append(f"    const bool ey = (y0 < REDGE) || (y0 + TY - 1 + REDGE > N1 - 1);")
+        # partial tiles (last tile of an axis not divisible by the tile) read through the
+        # gather variant: one-sided edge formulas there reach inward past the tile start
+        out.append(f"    const bool partial = (x0 + TX > N2) || (y0 + TY > N1);")
         # ring loads
         loads = []
         for i, L_ in enumerate(self.src_layout):
@@ -961,7 +966,7 @@ class KernelGen:
         out.append("            const IDX vox = ((IDX)z * N1 + y) * N2 + x;")
         names = {m: n for m, n in fns}
         if ("lds", False, False) in names:
-            out.append("            if (ez) ev_gather(p, A, s, z, y, x, ly, lx, y0, x0, vox);")
+            out.append("            if (ez || partial) ev_gather(p, A, s, z, y, x, ly, lx, y0, x0, vox);")
             out.append("            else if (!ex && !ey) ev_lds00(p, A, s, z, y, x, ly, lx, y0, x0, vox);")
             out.append("            else if (ex && !ey) ev_lds10(p, A, s, z, y, x, ly, lx, y0, x0, vox);")
             out.append("            else if (!ex) ev_lds01(p, A, s, z, y, x, ly, lx, y0, x0, vox);")

@@ -49,7 +49,7 @@ def test_c2_graph(shape):
 
 
 @pytest.mark.parametrize("sched", [Schedule(32, 4, 16, 256), Schedule(64, 8, 64, 256),
-                                   Schedule(128, 4, 32, 256), Schedule(64, 16, 8, 512)])
+                                   Schedule(128, 4, 32, 256), Schedule(64, 8, 8, 512)])
 def test_c2_schedules(sched):
     shape = (40, 45, 150)
     run_and_check(C2_EXPR, {"f": field(shape, 2), "u": vfield(shape)}, schedule=sched,
