@@ -1144,7 +1144,8 @@ class FusedExecutor:
             self._run_group(compiled[si], graph, results)
         return results
 
-    def _run_group(self, cg: CompiledGroup, graph: Graph, results: Dict[str, Any]) -> None:
+    def prepare_launch(self, cg: CompiledGroup, graph: Graph, results: Dict[str, Any]) -> "Launch":
+        """Buffers and packed arguments of one launch of ``cg`` (inputs read from ``results``)."""
         g = cg.group
         if cg.module is None:
             cg.module = load_module(cg.source)
@@ -1163,8 +1164,6 @@ class FusedExecutor:
         for o in g.outputs:
             ch = cg.gen.info[o].ch
             shape = (n0, n1, n2) + {1: (), 3: (3,), 9: (3, 3)}[ch]
-            if graph.node(o).op_name == "gradient" and graph.node(o).params.get("dim") is not None:
-                shape = (n0, n1, n2)
             dt = cg.gen.out_dtypes.get(o, torch.float32)
             outs.append(torch.empty(shape, dtype=dt, device=dev))
         tx, ty, zcn, nb = cg.nblocks()
@@ -1177,19 +1176,25 @@ class FusedExecutor:
             if not isinstance(t, torch.Tensor):
                 t = torch.tensor(float(t))
             scal.append(t.to(device=dev, dtype=torch.float64).reshape(1).contiguous())
-        args = pack_params(cg, srcs, outs, reds, dflags, scal, 0.0)
+        return Launch(cg, srcs, outs, reds, dflags, scal, nb)
+
+    def _run_group(self, cg: CompiledGroup, graph: Graph, results: Dict[str, Any]) -> None:
+        g = cg.group
+        n0, n1, n2 = g.grid
+        dev = self.device
+        ln = self.prepare_launch(cg, graph, results)
         stream = L.stream_ptr(dev)
-        cg.module.launch((nb, 1, 1), (cg.gen.sched.threads, 1, 1), args, 0, stream)
-        if ndet:
+        ln.launch(stream)
+        nb = ln.nblocks
+        if ln.dflags:
             flag = torch.empty(2, dtype=torch.float64, device=dev)
-            L.check(L.lib().sp_reduce_combine(L.REDUCE["any_small_diag"], dflags[0].data_ptr(), nb,
+            L.check(L.lib().sp_reduce_combine(L.REDUCE["any_small_diag"], ln.dflags[0].data_ptr(), nb,
                                               n0 * n1 * n2, flag.data_ptr(), stream), "det flag")
             if float(flag[1].item()) > 0.0:
-                args = pack_params(cg, srcs, outs, reds, dflags, scal, 1e-12)
-                cg.module.launch((nb, 1, 1), (cg.gen.sched.threads, 1, 1), args, 0, stream)
-        for o, t in zip(g.outputs, outs):
+                ln.launch(stream, det_reg=1e-12)
+        for o, t in zip(g.outputs, ln.outs):
             results[o] = t
-        for r, ws in zip(g.reductions, reds):
+        for r, ws in zip(g.reductions, ln.reds):
             node = graph.node(r)
             inp = node.input_ids[0]
             count = n0 * n1 * n2 * max(cg.gen.info[inp].ch, 1)
@@ -1198,3 +1203,40 @@ class FusedExecutor:
             L.check(L.lib().sp_reduce_combine(kind, ws.data_ptr(), nb, count, res.data_ptr(), stream),
                     f"combine {node.op_name}")
             results[r] = res if node.op_name == "min_max" else res.reshape(())
+
+
+@dataclass
+class Launch:
+    """One prepared launch of a compiled group (inputs, outputs, workspaces, packed args)."""
+    cg: CompiledGroup
+    srcs: List[torch.Tensor]
+    outs: List[torch.Tensor]
+    reds: List[torch.Tensor]
+    dflags: List[torch.Tensor]
+    scal: List[torch.Tensor]
+    nblocks: int
+
+    def args(self, det_reg: float = 0.0) -> bytes:
+        return pack_params(self.cg, self.srcs, self.outs, self.reds, self.dflags, self.scal, det_reg)
+
+    def launch(self, stream: Optional[int] = None, det_reg: float = 0.0) -> None:
+        self.cg.module.launch((self.nblocks, 1, 1), (self.cg.gen.sched.threads, 1, 1),
+                              self.args(det_reg), 0, stream if stream is not None else L.stream_ptr())
+
+    def time_ms(self, reps: int = 20, warmup: int = 3, rounds: int = 3) -> float:
+        """Min over ``rounds`` of the mean kernel time, HIP events on the launch stream."""
+        s = L.stream_ptr()
+        a = self.args()
+        grid, block = (self.nblocks, 1, 1), (self.cg.gen.sched.threads, 1, 1)
+        for _ in range(warmup):
+            self.cg.module.launch(grid, block, a, 0, s)
+        best = float("inf")
+        for _ in range(rounds):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                self.cg.module.launch(grid, block, a, 0, s)
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1) / reps)
+        return best
