@@ -84,7 +84,7 @@ class Schedule:
 DEFAULT_SCHEDULE = Schedule()
 # LDS per block kept within 64 KiB so that at least two blocks share a CU
 LDS_LIMIT = 64 * 1024
-FALLBACK_SCHEDULES = [Schedule(64, 4, 64, 256), Schedule(32, 8, 64, 256), Schedule(32, 4, 64, 256)]
+FALLBACK_SCHEDULES = [Schedule(64, 4, 64, 256), Schedule(32, 8, 64, 256), Schedule(64, 8, 64, 512)]
 
 
 # ------------------------------------------------------------------------------ node typing
@@ -349,6 +349,14 @@ def _flt(x: float) -> str:
     return r + "f"
 
 
+class _NeedRing(Exception):
+    """A queue-mode source was tapped off its own column at another plane."""
+
+    def __init__(self, si: int):
+        super().__init__(si)
+        self.si = si
+
+
 class _Emit:
     """Per-variant expression emitter with (node, position) memoisation.
 
@@ -360,9 +368,10 @@ class _Emit:
     AX_VAR = ("z", "y", "x")
     AX_N = ("N0", "N1", "N2")
 
-    def __init__(self, gen: "KernelGen", mode: Tuple):
+    def __init__(self, gen: "KernelGen", mode: Tuple, record: Optional[Dict[int, set]] = None):
         self.g = gen
         self.mode = mode
+        self.record = record
         self.lines: List[str] = []
         self.memo: Dict[Any, List[str]] = {}
         self.cmemo: Dict[str, str] = {}
@@ -427,6 +436,9 @@ class _Emit:
         return f"({c[1]} == 0)" if which == "lo" else f"({c[1]} == {self.AX_N[a]} - 1)"
 
     # -- taps --------------------------------------------------------------------------
+    def _is_origin(self, c, a: int) -> bool:
+        return c == ("I", 0) or (c[0] == "E" and c[1] == self.AX_VAR[a])
+
     def tap(self, si: int, ch: int, pos) -> str:
         key = ("tap", si, ch, pos)
         if key in self.memo:
@@ -439,18 +451,31 @@ class _Emit:
             cx = self._cvar(f"clampc({self.cexpr(pos[2], 2)}, N2)")
             idx = f"((((IDX){cz} * N1 + {cy}) * N2 + {cx}) * {src['C']} + {ch})"
             v = self.var(f"p.src{si}[{idx}]")
+            self.memo[key] = [v]
+            return v
+        kz = pos[0][1]
+        if self.record is not None and all(c[0] == "I" for c in pos):
+            self.record.setdefault(si, set()).add((pos[0][1], pos[1][1], pos[2][1]))
+        own = self._is_origin(pos[1], 1) and self._is_origin(pos[2], 2)
+        if src["mode"] == "queue":
+            if own and kz in (-1, 0, 1):
+                v = {-1: f"qm{si}[{ch}]", 0: f"q0{si}[{ch}]", 1: f"qp{si}[{ch}]"}[kz]
+                self.memo[key] = [v]
+                return v
+            if kz != 0 or not src["lds"]:
+                raise _NeedRing(si)
+            slot = "sq"
         else:
-            H, D = src["H"], src["D"]
-            kz = pos[0][1]
-            slot = f"((unsigned)(s{kz:+d}) % {D}u)" if kz else f"((unsigned)s % {D}u)"
-            rows = []
-            for a, (lvar, t0, T, N) in ((1, ("ly", "y0", "TY", "N1")), (2, ("lx", "x0", "TX", "N2"))):
-                c = pos[a]
-                if c[0] == "I":
-                    rows.append(f"({lvar}+{H + c[1]})")
-                else:
-                    rows.append(self._cvar(f"lmap({c[1]}, {t0}, {T}, {H}, {N})"))
-            v = self.var(f"lds_s{si}[{slot}][{ch}][{rows[0]}][{rows[1]}]")
+            slot = f"((unsigned)(s{kz:+d}) % {src['D']}u)" if kz else f"((unsigned)s % {src['D']}u)"
+        H = src["H"]
+        rows = []
+        for a, (lvar, t0, T, N) in ((1, ("ly", "y0", "TY", "N1")), (2, ("lx", "x0", "TX", "N2"))):
+            c = pos[a]
+            if c[0] == "I":
+                rows.append(f"({lvar}+{H + c[1]})")
+            else:
+                rows.append(self._cvar(f"lmap({c[1]}, {t0}, {T}, {H}, {N})"))
+        v = self.var(f"lds_s{si}[{slot}][{ch}][{rows[0]}][{rows[1]}]")
         self.memo[key] = [v]
         return v
 
@@ -765,7 +790,18 @@ def source_reach(graph: Graph, group: Group) -> Dict[str, Tuple[int, int, int]]:
 
 
 class KernelGen:
-    """Builds the HIP source of one fused group for one schedule."""
+    """Builds the HIP source of one fused group for one schedule.
+
+    Per field source the generator picks a mode from the taps the interior code makes:
+      * "queue": every tap off the current plane sits on the voxel's own column (and the
+        interior reach is 1) -- the z-neighbours live in a per-thread register queue and LDS
+        holds only the current plane (two slots: current + next); centre taps need no LDS;
+      * "ring": an LDS ring of 2*hz+2 planes (mixed z/in-plane taps, e.g. hessians).
+    Each thread owns V voxels per plane (column tid % TX, rows tid / TX + k * NT / TX); it
+    loads its own cells plus a share of the halo, with plane-relative addresses computed
+    once per block, into one of two register buffers.  The z-loop is unrolled by two so
+    that each buffer's loads have two iterations to land.
+    """
 
     def __init__(self, graph: Graph, group: Group, info: Dict[str, Info], sched: Schedule,
                  out_dtypes: Dict[str, torch.dtype], src_meta: Dict[str, Tuple[int, ...]]):
@@ -777,18 +813,70 @@ class KernelGen:
         reach = source_reach(graph, group)
         self.hz = max([r[0] for r in reach.values()] + [0])
         self.redge = max([r[1] for r in reach.values()] + [0])
-        D = 2 * self.hz + 2
-        self.src_layout = [{"C": info[s].ch, "H": reach[s][2], "D": D} for s in group.sources]
-        # point-wise-only groups read their sources directly (no ring, no halo)
         self.use_lds = self.hz > 0 and len(group.sources) > 0
+        if sched.threads % sched.tx != 0 or sched.ty % (sched.threads // sched.tx) != 0:
+            raise ValueError(f"schedule {sched} does not map threads onto whole tile rows")
+        self.rp = sched.threads // sched.tx          # rows per pass
+        self.v = sched.ty // self.rp                 # voxels per thread per plane
+        self.src_layout = [{"C": info[s].ch, "H": reach[s][2], "D": 2 * self.hz + 2,
+                            "mode": "ring", "lds": True} for s in group.sources]
+        self._choose_modes()
+
+    # -- source modes ----------------------------------------------------------------------
+    def _choose_modes(self) -> None:
+        if not self.use_lds:
+            return
+        rec: Dict[int, set] = {}
+        e = _Emit(self, ("lds", False, False), record=rec)
+        self._walk(e)
+        for si, L_ in enumerate(self.src_layout):
+            taps = rec.get(si, set())
+            zoff = all(ky == 0 and kx == 0 for kz, ky, kx in taps if kz != 0)
+            inplane = any((ky, kx) != (0, 0) for kz, ky, kx in taps)
+            if self.hz == 1 and zoff:
+                L_.update(mode="queue", D=2, lds=inplane)
+                if not inplane:
+                    L_["H"] = 0
+        # the edge variants may still need a ring (e.g. a wrapped coordinate off the column)
+        while True:
+            try:
+                for m in self._variants():
+                    self._walk(_Emit(self, m))
+                break
+            except _NeedRing as nr:
+                L_ = self.src_layout[nr.si]
+                L_.update(mode="ring", D=2 * self.hz + 2, lds=True)
+
+    def _variants(self):
+        v = [("gather",)]
+        if self.use_lds:
+            v += [("lds", ex, ey) for ex in (False, True) for ey in (False, True)]
+        return v
+
+    def _walk(self, e: "_Emit") -> None:
+        pos = e.origin()
+        for nid in self.group.outputs:
+            e.val(nid, pos)
+        for nid in self.group.reductions:
+            e.val(self.graph.node(nid).input_ids[0], pos)
+        for nid in self._det_inputs():
+            e.val(nid, pos)
+
+    def queue_sources(self) -> List[int]:
+        return [i for i, L_ in enumerate(self.src_layout) if self.use_lds and L_["mode"] == "queue"]
+
+    def lds_sources(self) -> List[int]:
+        return [i for i, L_ in enumerate(self.src_layout) if self.use_lds and L_["lds"]]
 
     def lds_bytes(self) -> int:
         s = self.sched
         tot = 0
-        for L_ in (self.src_layout if self.use_lds else []):
+        for i in self.lds_sources():
+            L_ = self.src_layout[i]
             tot += L_["D"] * L_["C"] * (s.ty + 2 * L_["H"]) * (s.tx + 2 * L_["H"]) * 4
-        return tot + 3 * (s.threads // 64) * 4 * 2
+        return tot + 4 * (s.threads // 64) * 4 + 16
 
+    # -- per-voxel code ------------------------------------------------------------------------
     def _emit_variant(self, mode) -> Tuple[str, str]:
         e = _Emit(self, mode)
         pos = e.origin()
@@ -822,10 +910,12 @@ class KernelGen:
                         f"fabsf({vals[8]}) < 1e-12f) A.dflag{ki} = 1.0f;")
         name = "ev_gather" if mode[0] == "gather" else f"ev_lds{int(mode[1])}{int(mode[2])}"
         code = "\n        ".join(e.lines + body)
-        fn = (f"__device__ __forceinline__ void {name}(const Params& p, Acc& A, int s, int z, int y, int x,\n"
-              f"        int ly, int lx, int y0, int x0, IDX vox) {{\n"
-              f"    {'' if self.group.dscalars == [] else ' '.join(f'const float sc{i} = A.sc{i};' for i in range(len(self.group.dscalars)))}\n"
-              f"        {code}\n}}\n")
+        qargs = "".join(f", const float* qm{i}, const float* q0{i}, const float* qp{i}"
+                        for i in self.queue_sources())
+        sc = " ".join(f"const float sc{i} = A.sc{i};" for i in range(len(self.group.dscalars)))
+        fn = (f"__device__ __forceinline__ void {name}(const Params& p, Acc& A, int s, int sq, int z,\n"
+              f"        int y, int x, int ly, int lx, int y0, int x0, IDX vox{qargs}) {{\n"
+              f"    {sc}\n        {code}\n}}\n")
         return name, fn
 
     def _det_inputs(self) -> List[str]:
@@ -848,19 +938,13 @@ class KernelGen:
             lines.append((CSRC / hdr).read_text().replace("#pragma once", ""))
         lines.append(f"#define N0 {n0}\n#define N1 {n1}\n#define N2 {n2}\n"
                      f"#define TX {s.tx}\n#define TY {s.ty}\n#define ZC {s.zc}\n#define NT {s.threads}\n"
-                     f"#define HZ {self.hz}\n#define REDGE {self.redge}\ntypedef {idx_t} IDX;\n")
-        # params
-        pf = []
-        for i in range(ns):
-            pf.append(f"    const float* __restrict__ src{i};")
-        for i in range(no):
-            pf.append(f"    void* out{i};")
-        for i in range(nr):
-            pf.append(f"    double* red{i};")
-        for i in range(ndet):
-            pf.append(f"    double* dflag{i};")
-        for i in range(nd):
-            pf.append(f"    const double* scal{i};")
+                     f"#define HZ {self.hz}\n#define REDGE {self.redge}\n#define RP {self.rp}\n"
+                     f"#define VPT {self.v}\ntypedef {idx_t} IDX;\n")
+        pf = [f"    const float* __restrict__ src{i};" for i in range(ns)]
+        pf += [f"    void* out{i};" for i in range(no)]
+        pf += [f"    double* red{i};" for i in range(nr)]
+        pf += [f"    double* dflag{i};" for i in range(ndet)]
+        pf += [f"    const double* scal{i};" for i in range(nd)]
         pf.append("    int tiles_x, tiles_y, zchunks, nblocks;\n    float det_reg; int pad_;")
         lines.append("struct Params {\n" + "\n".join(pf) + "\n};\n")
         acc = []
@@ -868,126 +952,206 @@ class KernelGen:
             op = self.graph.node(nid).op_name
             ty = {"variance": "RVar", "min_max": "RMinMax"}.get(op, "RSum")
             acc.append(f"    {ty} r{ri};")
-        for i in range(ndet):
-            acc.append(f"    float dflag{i};")
-        for i in range(nd):
-            acc.append(f"    float sc{i};")
+        acc += [f"    float dflag{i};" for i in range(ndet)]
+        acc += [f"    float sc{i};" for i in range(nd)]
         acc.append("    int dummy_;")
         lines.append("struct Acc {\n" + "\n".join(acc) + "\n};\n")
-        # LDS
-        if self.use_lds:
-            for i, L_ in enumerate(self.src_layout):
-                lines.append(f"__shared__ float lds_s{i}[{L_['D']}][{L_['C']}][TY + {2 * L_['H']}][TX + {2 * L_['H']}];")
-        variants = [("gather",)]
-        if self.use_lds:
-            variants += [("lds", ex, ey) for ex in (False, True) for ey in (False, True)]
+        for i in self.lds_sources():
+            L_ = self.src_layout[i]
+            lines.append(f"__shared__ float lds_s{i}[{L_['D']}][{L_['C']}][TY + {2 * L_['H']}][TX + {2 * L_['H']}];")
         fns = []
-        for m in variants:
+        for m in self._variants():
             name, fn = self._emit_variant(m)
             fns.append((m, name))
             lines.append(fn)
         lines.append(self._kernel(fns, ndet))
         return "\n".join(lines)
 
+    # -- kernel skeleton ---------------------------------------------------------------------
     def _kernel(self, fns, ndet: int) -> str:
         s, g = self.sched, self.group
-        ns = len(g.sources)
-        out = []
-        out.append("extern \"C\" __global__ void __launch_bounds__(NT) fused_kernel(const Params p) {")
-        out.append("    __shared__ float red_scratch[4 * NT / 64 + 4];")
-        out.append("    const int tid = threadIdx.x;")
-        # XCD-aware block remap: XCD k (hardware block id % 8) walks a contiguous tile range.
-        out.append("    const int b = blockIdx.x, nb = p.nblocks, per = nb >> 3, rem = nb & 7;")
-        out.append("    const int xcd = b & 7, lb = xcd * per + (xcd < rem ? xcd : rem) + (b >> 3);")
-        out.append("    const int tx_i = lb % p.tiles_x, t_ = lb / p.tiles_x;")
-        out.append("    const int ty_i = t_ % p.tiles_y, zc_i = t_ / p.tiles_y;")
-        out.append("    const int x0 = tx_i * TX, y0 = ty_i * TY;")
-        out.append("    const int zb = zc_i * ZC, ze = (zb + ZC < N0) ? zb + ZC : N0;")
-        out.append("    Acc A;")
-        for ri, nid in enumerate(g.reductions):
-            out.append(f"    A.r{ri}.init();")
+        out: List[str] = []
+        A = out.append
+        A('extern "C" __global__ void __launch_bounds__(NT) fused_kernel(const Params p) {')
+        A("    __shared__ float red_scratch[4 * NT / 64 + 4];")
+        A("    const int tid = threadIdx.x;")
+        # XCD-aware block remap: XCD k (hardware block id % 8) walks a contiguous tile range
+        A("    const int b = blockIdx.x, nb = p.nblocks, per = nb >> 3, rem = nb & 7;")
+        A("    const int xcd = b & 7, lb = xcd * per + (xcd < rem ? xcd : rem) + (b >> 3);")
+        A("    const int tx_i = lb % p.tiles_x, t_ = lb / p.tiles_x;")
+        A("    const int ty_i = t_ % p.tiles_y, zc_i = t_ / p.tiles_y;")
+        A("    const int x0 = tx_i * TX, y0 = ty_i * TY;")
+        A("    const int zb = zc_i * ZC, ze = (zb + ZC < N0) ? zb + ZC : N0;")
+        A("    const int lx = tid % TX, ly0 = tid / TX;")
+        A("    Acc A;")
+        for ri in range(len(g.reductions)):
+            A(f"    A.r{ri}.init();")
         for i in range(ndet):
-            out.append(f"    A.dflag{i} = 0.0f;")
+            A(f"    A.dflag{i} = 0.0f;")
         for i in range(len(g.dscalars)):
-            out.append(f"    A.sc{i} = (float)p.scal{i}[0];")
-        out.append(f"    const bool ex = (x0 < REDGE) || (x0 + TX - 1 + REDGE > N2 - 1);")
-        out.append(f"    const bool ey = (y0 < REDGE) || (y0 + TY - 1 + REDGE > N1 - 1);")
-        # partial tiles (last tile of an axis not divisible by the tile) read through the
-        # gather variant: one-sided edge formulas there reach inward past the tile start
-        out.append(f"    const bool partial = (x0 + TX > N2) || (y0 + TY > N1);")
-        # ring loads
-        loads = []
-        for i, L_ in enumerate(self.src_layout):
-            C, H = L_["C"], L_["H"]
-            RW, RH = s.tx + 2 * H, s.ty + 2 * H
-            RL = RW * C
-            TOT = RH * RL
-            K = (TOT + s.threads - 1) // s.threads
-            loads.append((i, C, H, RW, RH, RL, TOT, K, L_["D"]))
-        if self.use_lds:
-            for (i, C, H, RW, RH, RL, TOT, K, D) in loads:
-                out.append(f"    float pf{i}[{K}];")
-            # macro-like lambdas
-            out.append("    auto fetch_plane = [&](int zg) {")
-            out.append("        const int zw = wrapm(zg, N0);")
-            for (i, C, H, RW, RH, RL, TOT, K, D) in loads:
-                out.append(f"        #pragma unroll\n        for (int k = 0; k < {K}; ++k) {{")
-                out.append(f"            const int e = tid + k * NT;")
-                out.append(f"            if ({TOT} % NT == 0 || e < {TOT}) {{")
-                out.append(f"                const int row = e / {RL}, r = e - row * {RL};")
-                out.append(f"                const int xi = r / {C}, ch = r - xi * {C};")
-                out.append(f"                const int gy = wrapm(y0 - {H} + row, N1), gx = wrapm(x0 - {H} + xi, N2);")
-                out.append(f"                pf{i}[k] = p.src{i}[(((IDX)zw * N1 + gy) * N2 + gx) * {C} + ch];")
-                out.append("            }")
-                out.append("        }")
-            out.append("    };")
-            out.append("    auto store_plane = [&](int slot) {")
-            for (i, C, H, RW, RH, RL, TOT, K, D) in loads:
-                out.append(f"        #pragma unroll\n        for (int k = 0; k < {K}; ++k) {{")
-                out.append(f"            const int e = tid + k * NT;")
-                out.append(f"            if ({TOT} % NT == 0 || e < {TOT}) {{")
-                out.append(f"                const int row = e / {RL}, r = e - row * {RL};")
-                out.append(f"                const int xi = r / {C}, ch = r - xi * {C};")
-                out.append(f"                lds_s{i}[(unsigned)slot % {D}u][ch][row][xi] = pf{i}[k];")
-                out.append("            }")
-                out.append("        }")
-            out.append("    };")
-            out.append("    for (int k = -HZ; k <= HZ; ++k) { fetch_plane(zb + k); store_plane(k + HZ); }")
-            out.append("    fetch_plane(zb + HZ + 1);")
-            out.append("    __syncthreads();")
-        out.append("    for (int z = zb; z < ze; ++z) {")
-        out.append("        const int s = z - zb + HZ;")
-        out.append("        const bool ez = (z < REDGE) || (z + REDGE > N0 - 1);")
-        out.append("        #pragma unroll 1")
-        out.append("        for (int v = tid; v < TX * TY; v += NT) {")
-        out.append("            const int ly = v / TX, lx = v - ly * TX;")
-        out.append("            const int y = y0 + ly, x = x0 + lx;")
-        out.append("            if (y >= N1 || x >= N2) continue;")
-        out.append("            const IDX vox = ((IDX)z * N1 + y) * N2 + x;")
+            A(f"    A.sc{i} = (float)p.scal{i}[0];")
+        A("    const bool ex = (x0 < REDGE) || (x0 + TX - 1 + REDGE > N2 - 1);")
+        A("    const bool ey = (y0 < REDGE) || (y0 + TY - 1 + REDGE > N1 - 1);")
+        # partial tiles (last tile of an axis not divisible by the tile) use the gather
+        # variant: one-sided edge formulas there reach inward past the tile start
+        A("    const bool partial = (x0 + TX > N2) || (y0 + TY > N1);")
         names = {m: n for m, n in fns}
-        if ("lds", False, False) in names:
-            out.append("            if (ez || partial) ev_gather(p, A, s, z, y, x, ly, lx, y0, x0, vox);")
-            out.append("            else if (!ex && !ey) ev_lds00(p, A, s, z, y, x, ly, lx, y0, x0, vox);")
-            out.append("            else if (ex && !ey) ev_lds10(p, A, s, z, y, x, ly, lx, y0, x0, vox);")
-            out.append("            else if (!ex) ev_lds01(p, A, s, z, y, x, ly, lx, y0, x0, vox);")
-            out.append("            else ev_lds11(p, A, s, z, y, x, ly, lx, y0, x0, vox);")
-        else:
-            out.append("            ev_gather(p, A, s, z, y, x, ly, lx, y0, x0, vox);")
-        out.append("        }")
+        qs = self.queue_sources()
+        lds = self.lds_sources()
         if self.use_lds:
-            out.append("        store_plane(s + HZ + 1);")
-            out.append("        __syncthreads();")
-            out.append("        if (z + 1 < ze) fetch_plane(z + HZ + 2);")
-        out.append("    }")
+            self._emit_loaders(A)
+            self._emit_prologue(A)
+        A("    for (int z = zb; z < ze; z += 2) {")
+        self._emit_step(A, names, "A", 0)
+        A("        if (z + 1 >= ze) break;")
+        A("        {")
+        A("        const int z_ = z + 1;")
+        self._emit_step(A, names, "B", 1, zname="z_")
+        A("        }")
+        A("    }")
         for ri, nid in enumerate(g.reductions):
             op = self.graph.node(nid).op_name
             fn = {"variance": "block_var", "min_max": "block_mm"}.get(op, "block_sum")
-            out.append(f"    {fn}(A.r{ri}, red_scratch, p.red{ri} + (i64)blockIdx.x * 4);")
+            A(f"    {fn}(A.r{ri}, red_scratch, p.red{ri} + (i64)blockIdx.x * 4);")
         for i in range(ndet):
-            out.append(f"    {{ RMinMax fm; fm.lo = INFINITY; fm.hi = A.dflag{i} > 0.0f ? 1.0f : -INFINITY;"
-                       f" block_mm(fm, red_scratch, p.dflag{i} + (i64)blockIdx.x * 4); }}")
-        out.append("}")
+            A(f"    {{ RMinMax fm; fm.lo = INFINITY; fm.hi = A.dflag{i} > 0.0f ? 1.0f : -INFINITY;"
+              f" block_mm(fm, red_scratch, p.dflag{i} + (i64)blockIdx.x * 4); }}")
+        A("}")
         return "\n".join(out)
+
+    def _halo_count(self, i: int) -> int:
+        L_ = self.src_layout[i]
+        if not L_["lds"]:
+            return 0
+        H, C = L_["H"], L_["C"]
+        W = self.sched.tx + 2 * H
+        return (2 * H * W + 2 * H * self.sched.ty) * C
+
+    def _emit_loaders(self, A) -> None:
+        s = self.sched
+        for i, L_ in enumerate(self.src_layout):
+            C, H = L_["C"], L_["H"]
+            W, RH = s.tx + 2 * H, s.ty + 2 * H
+            HT = self._halo_count(i)
+            KH = (HT + s.threads - 1) // s.threads
+            L_["KH"] = KH
+            # own cells: plane-relative offsets (clamped so partial tiles stay in bounds)
+            A(f"    int og{i}[VPT];")
+            A("    #pragma unroll")
+            A("    for (int k = 0; k < VPT; ++k)")
+            A(f"        og{i}[k] = (clampc(y0 + ly0 + k * RP, N1) * N2 + clampc(x0 + lx, N2)) * {C};")
+            if KH:
+                A(f"    int hg{i}[{KH}], hl{i}[{KH}];")
+                A("    #pragma unroll")
+                A(f"    for (int j = 0; j < {KH}; ++j) {{")
+                A(f"        const int e = tid + j * NT;")
+                A(f"        int r, col, ch;")
+                top = H * W * C
+                mid = s.ty * 2 * H * C
+                A(f"        if (e < {top}) {{ r = e / {W * C}; const int q = e - r * {W * C}; col = q / {C}; ch = q - col * {C}; }}")
+                A(f"        else if (e < {top + mid}) {{ const int e2 = e - {top}; r = {H} + e2 / {2 * H * C};"
+                  f" const int q = e2 - (r - {H}) * {2 * H * C}; const int jj = q / {C}; ch = q - jj * {C};"
+                  f" col = jj < {H} ? jj : TX + jj; }}")
+                A(f"        else {{ const int e3 = e - {top + mid}; r = TY + {H} + e3 / {W * C};"
+                  f" const int q = e3 - (r - TY - {H}) * {W * C}; col = q / {C}; ch = q - col * {C}; }}")
+                A(f"        hg{i}[j] = (wrapm(y0 - {H} + r, N1) * N2 + wrapm(x0 - {H} + col, N2)) * {C} + ch;")
+                A(f"        hl{i}[j] = e < {HT} ? (ch * {RH} + r) * {W} + col : -1;")
+                A("    }")
+            for buf in ("A", "B"):
+                A(f"    float {buf}o{i}[VPT][{C}];")
+                if KH:
+                    A(f"    float {buf}h{i}[{KH}];")
+            if L_["mode"] == "queue":
+                A(f"    float qm{i}[VPT][{C}], q0{i}[VPT][{C}];")
+
+    def _fetch(self, A, i: int, buf: str, zexpr: str, indent: str = "    ") -> None:
+        L_ = self.src_layout[i]
+        C, KH = L_["C"], L_.get("KH", 0)
+        A(f"{indent}{{ const IDX pb = (IDX)wrapm({zexpr}, N0) * (N1 * N2 * {C});")
+        A(f"{indent}  #pragma unroll")
+        A(f"{indent}  for (int k = 0; k < VPT; ++k)")
+        A(f"{indent}    #pragma unroll")
+        A(f"{indent}    for (int c = 0; c < {C}; ++c) {buf}o{i}[k][c] = p.src{i}[pb + og{i}[k] + c];")
+        if KH:
+            A(f"{indent}  #pragma unroll")
+            A(f"{indent}  for (int j = 0; j < {KH}; ++j) if (hl{i}[j] >= 0) {buf}h{i}[j] = p.src{i}[pb + hg{i}[j]];")
+        A(f"{indent}}}")
+
+    def _store(self, A, i: int, buf: str, slot: str, indent: str = "    ") -> None:
+        L_ = self.src_layout[i]
+        if not L_["lds"]:
+            return
+        C, H, KH = L_["C"], L_["H"], L_.get("KH", 0)
+        W, RH = self.sched.tx + 2 * H, self.sched.ty + 2 * H
+        A(f"{indent}{{ float* ls = &lds_s{i}[{slot}][0][0][0];")
+        A(f"{indent}  #pragma unroll")
+        A(f"{indent}  for (int k = 0; k < VPT; ++k)")
+        A(f"{indent}    #pragma unroll")
+        A(f"{indent}    for (int c = 0; c < {C}; ++c) ls[(c * {RH} + ly0 + k * RP + {H}) * {W} + lx + {H}] = {buf}o{i}[k][c];")
+        if KH:
+            A(f"{indent}  #pragma unroll")
+            A(f"{indent}  for (int j = 0; j < {KH}; ++j) if (hl{i}[j] >= 0) ls[hl{i}[j]] = {buf}h{i}[j];")
+        A(f"{indent}}}")
+
+    def _lead(self, i: int) -> int:
+        return 1 if self.src_layout[i]["mode"] == "queue" else self.hz + 1
+
+    def _emit_prologue(self, A) -> None:
+        for i, L_ in enumerate(self.src_layout):
+            C = L_["C"]
+            if L_["mode"] == "queue":
+                self._fetch(A, i, "A", "zb - 1")
+                A(f"    for (int k = 0; k < VPT; ++k) for (int c = 0; c < {C}; ++c) qm{i}[k][c] = Ao{i}[k][c];")
+                self._fetch(A, i, "A", "zb")
+                A(f"    for (int k = 0; k < VPT; ++k) for (int c = 0; c < {C}; ++c) q0{i}[k][c] = Ao{i}[k][c];")
+                self._store(A, i, "A", "0")
+            else:
+                A("    for (int k = -HZ; k <= HZ; ++k) {")
+                self._fetch(A, i, "A", "zb + k", "        ")
+                self._store(A, i, "A", f"(unsigned)(k + HZ) % {L_['D']}u", "        ")
+                A("    }")
+            self._fetch(A, i, "A", f"zb + {self._lead(i)}")
+            self._fetch(A, i, "B", f"zb + 1 + {self._lead(i)}")
+        A("    __syncthreads();")
+
+    def _emit_step(self, A, names, buf: str, sq: int, zname: str = "z") -> None:
+        ind = "        "
+        z = zname
+        A(f"{ind}{{ const int s = {z} - zb + HZ;")
+        A(f"{ind}  const bool ez = ({z} < REDGE) || ({z} + REDGE > N0 - 1);")
+        A(f"{ind}  #pragma unroll")
+        A(f"{ind}  for (int k = 0; k < VPT; ++k) {{")
+        A(f"{ind}    const int ly = ly0 + k * RP, y = y0 + ly, x = x0 + lx;")
+        A(f"{ind}    if (y < N1 && x < N2) {{")
+        A(f"{ind}      const IDX vox = ((IDX){z} * N1 + y) * N2 + x;")
+        qa = "".join(f", qm{i}[k], q0{i}[k], {buf}o{i}[k]" for i in self.queue_sources())
+        call = f"(p, A, s, {sq}, {z}, y, x, ly, lx, y0, x0, vox{qa});"
+        if ("lds", False, False) in names:
+            A(f"{ind}      if (ez || partial) ev_gather{call}")
+            A(f"{ind}      else if (!ex && !ey) ev_lds00{call}")
+            A(f"{ind}      else if (ex && !ey) ev_lds10{call}")
+            A(f"{ind}      else if (!ex) ev_lds01{call}")
+            A(f"{ind}      else ev_lds11{call}")
+        else:
+            A(f"{ind}      ev_gather{call}")
+        A(f"{ind}    }}")
+        A(f"{ind}  }}")
+        if self.use_lds:
+            for i, L_ in enumerate(self.src_layout):
+                slot = str(sq ^ 1) if L_["mode"] == "queue" else f"(unsigned)(s + HZ + 1) % {L_['D']}u"
+                self._store(A, i, buf, slot, ind + "  ")
+            A(f"{ind}  __syncthreads();")
+            for i in self.queue_sources():
+                C = self.src_layout[i]["C"]
+                A(f"{ind}  #pragma unroll")
+                A(f"{ind}  for (int k = 0; k < VPT; ++k)")
+                A(f"{ind}    #pragma unroll")
+                A(f"{ind}    for (int c = 0; c < {C}; ++c) {{ qm{i}[k][c] = q0{i}[k][c]; q0{i}[k][c] = {buf}o{i}[k][c]; }}")
+            A(f"{ind}  if ({z} + 2 < ze) {{")
+            for i in range(len(self.src_layout)):
+                self._fetch(A, i, buf, f"{z} + 2 + {self._lead(i)}", ind + "    ")
+            A(f"{ind}  }}")
+        A(f"{ind}}}")
 
 
 # ------------------------------------------------------------------------------ compiled plans
