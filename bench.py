@@ -47,40 +47,14 @@ def time_steps(fn, steps: int, warmup: int):
     return (time.perf_counter() - t0) / steps
 
 
-def kernel_ms(cg, args, nb, threads, reps: int) -> float:
-    """Average duration of the fused kernel alone, HIP events on the launch stream."""
-    import torch
-    from stenpy_amd import _lib as L
-    s = L.stream_ptr()
-    for _ in range(3):
-        cg.module.launch((nb, 1, 1), (threads, 1, 1), args, 0, s)
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record()
-    for _ in range(reps):
-        cg.module.launch((nb, 1, 1), (threads, 1, 1), args, 0, s)
-    stop.record()
-    torch.cuda.synchronize()
-    return start.elapsed_time(stop) / reps
-
-
-def fused_kernel_args(rt, g, res, sink, reds, out_dtype):
-    """Compiled group + packed args reproducing one evaluation's main kernel launch."""
-    import torch
-    from stenpy_amd import fused as F
+def group_launch(rt, g, res, sink, out_dtype=None):
+    """The prepared launch (main + boundary kernel) of the fused group producing ``sink``."""
     ex = rt.fused
-    plan = next(iter(ex._plans.values())) if len(ex._plans) == 1 else None
-    key = [k for k, v in ex._plans.items()]
-    for k in key:
-        steps, info, comp = ex._plans[k]
+    for steps, info, comp in ex._plans.values():
         for si, cg in comp.items():
-            if sink in cg.group.outputs and (out_dtype is None or cg.gen.out_dtypes.get(sink) == out_dtype):
-                srcs = [res[s] for s in cg.group.sources]
-                outs = [res[o] for o in cg.group.outputs]
-                tx, ty, zc, nb = cg.nblocks()
-                ws = [torch.empty(nb * 4, dtype=torch.float64, device=srcs[0].device) for _ in cg.group.reductions]
-                args = F.pack_params(cg, srcs, outs, ws, [], [], 0.0)
-                return cg, args, nb
-    raise RuntimeError("fused kernel not found")
+            if sink in cg.group.outputs and cg.gen.out_dtypes.get(sink) == out_dtype:
+                return cg, ex.prepare_launch(cg, g, res)
+    raise RuntimeError("fused group not found")
 
 
 def cpu_baseline(n: int, planes: int):
@@ -150,8 +124,8 @@ def main() -> int:
     t_bf16 = time_steps(fused_bf16, args.steps, args.warmup)
     t_plain = time_steps(plain, max(3, args.steps // 4), 1)
 
-    cg, kargs, nb = fused_kernel_args(rt, g, res, sink, reds, None)
-    k_ms = kernel_ms(cg, kargs, nb, cg.gen.sched.threads, max(10, args.steps))
+    cg, ln = group_launch(rt, g, res, sink)
+    k_ms = ln.time_ms(reps=max(10, args.steps))
     alg_bytes = 20 * nvox
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
     traffic = None
@@ -178,6 +152,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel_ms": round(k_ms, 4),
+                     "kernels": "fused_kernel + fused_boundary (z-edge planes, partial tiles)",
                      "algorithmic_bytes_per_launch": alg_bytes},
     }
     if not args.no_cpu:

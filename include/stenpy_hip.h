@@ -148,6 +148,14 @@ int sp_tensor3(int kind, const sp_tensor* h, sp_tensor* out, const double* flag,
 int sp_gen_fourier(sp_tensor* out, const int64_t n[3], int64_t z0, const int32_t* k,
                    const double* a, const double* phi, uint64_t noise_seed, void* stream);
 
+/* Host (CPU, OpenMP) twin of sp_gen_fourier for out-of-core slabs: writes planes
+ * [z0, z0 + nz) into host memory ``out`` (pinned for fast copies) with ``threads`` threads
+ * (0 = all).  Separable FP64 evaluation; FP32 values agree with the device generator up to a
+ * rare one-ulp difference. */
+int sp_host_gen_fourier(float* out, const int64_t n[3], int64_t z0, int64_t nz,
+                        const int32_t* k, const double* a, const double* phi, uint64_t noise_seed,
+                        int threads);
+
 /* ---- run-time compiled kernels (fused plans) ----------------------------------------------
  * The fused executor generates CDNA4 HIP source per graph and compiles it with hipRTC. */
 typedef struct sp_module sp_module;

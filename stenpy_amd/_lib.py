@@ -96,6 +96,8 @@ def _declare(lib: C.CDLL) -> None:
         "sp_module_launch": ([V, C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.c_uint, V, V, Z], I),
         "sp_module_attributes": ([V, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)], I),
         "sp_module_unload": ([V], None),
+        "sp_host_gen_fourier": ([V, C.POINTER(C.c_int64), C.c_int64, C.c_int64, C.POINTER(C.c_int32),
+                                 C.POINTER(D), C.POINTER(D), C.c_uint64, I], I),
         "sp_gemm_bf16": ([I, I, I, I, I, V, I, V, I, V, I, C.c_float, I, V], I),
         "sp_mlp_bias_act": ([V, V, I, I, I, V, V, V], I),
         "sp_mlp_relu_grad": ([V, V, C.c_int64, V, V, V], I),
@@ -326,3 +328,18 @@ class Module:
                 lib().sp_module_unload(self.handle)
         except Exception:
             pass
+
+
+def host_gen_fourier(out_ptr: int, shape, z0: int, nz: int, ks: np.ndarray, amps: np.ndarray,
+                     phis: np.ndarray, seed: int, threads: int = 0) -> None:
+    """Fill host memory at ``out_ptr`` with planes [z0, z0+nz) of the seeded field."""
+    n = (C.c_int64 * 3)(*shape)
+    k = np.ascontiguousarray(ks, dtype=np.int32).reshape(-1)
+    a = np.ascontiguousarray(amps, dtype=np.float64)
+    p = np.ascontiguousarray(phis, dtype=np.float64)
+    rc = lib().sp_host_gen_fourier(out_ptr, n, int(z0), int(nz), k.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   a.ctypes.data_as(C.POINTER(C.c_double)),
+                                   p.ctypes.data_as(C.POINTER(C.c_double)),
+                                   C.c_uint64(seed & ((1 << 64) - 1)), int(threads))
+    if rc != 0:
+        raise ValueError("host_gen_fourier: bad arguments")

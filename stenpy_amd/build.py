@@ -46,14 +46,27 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+def _compile_host(src: Path) -> Path:
+    """Host-only C++ (the slab generator): g++ with OpenMP and per-ISA clones."""
+    obj = OBJ / (src.stem + ".o")
+    cmd = ["g++", "-O3", "-fPIC", "-std=c++17", "-fopenmp", f"-I{ROOT / 'include'}", "-c",
+           str(src), "-o", str(obj)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src.name}:\n{r.stderr[-6000:]}")
+    return obj
+
+
 def build(verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
     sources = sorted(CSRC.glob("*.hip"))
-    jobs = min(len(sources), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)), 16)
+    host_sources = sorted(CSRC.glob("*.cpp"))
+    jobs = min(len(sources) + len(host_sources), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)), 16)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(_compile, sources))
+        futs = [ex.submit(_compile, s) for s in sources] + [ex.submit(_compile_host, s) for s in host_sources]
+        objs = [f.result() for f in futs]
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-fopenmp", "-o", str(tmp),
            *map(str, objs), "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

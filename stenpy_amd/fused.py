@@ -853,6 +853,71 @@ class KernelGen:
             v += [("lds", ex, ey) for ex in (False, True) for ey in (False, True)]
         return v
 
+    # -- boundary kernel -------------------------------------------------------------------------
+    def boundary_boxes(self) -> List[Tuple[int, int, int, int, int, int]]:
+        """Disjoint boxes (z0, z1, y0, y1, x0, x1) the main kernel leaves out."""
+        if not self.use_lds:
+            return []
+        n0, n1, n2 = self.grid
+        r = self.redge
+        s = self.sched
+        xp = (n2 // s.tx) * s.tx          # start of a partial x tile (== n2 if none)
+        yp = (n1 // s.ty) * s.ty
+        boxes = []
+        zlo, zhi = min(r, n0), max(n0 - r, min(r, n0))
+        if zlo > 0:
+            boxes.append((0, zlo, 0, n1, 0, n2))
+        if zhi < n0:
+            boxes.append((zhi, n0, 0, n1, 0, n2))
+        if zhi > zlo:
+            if xp < n2:
+                boxes.append((zlo, zhi, 0, n1, xp, n2))
+            if yp < n1:
+                boxes.append((zlo, zhi, yp, n1, 0, xp))
+        return [b for b in boxes if b[1] > b[0] and b[3] > b[2] and b[5] > b[4]]
+
+    def boundary_blocks(self) -> int:
+        tot = sum((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]) for b in self.boundary_boxes())
+        return (tot + 255) // 256
+
+    def _boundary_kernel(self, ndet: int) -> str:
+        g = self.group
+        boxes = self.boundary_boxes()
+        out: List[str] = []
+        A = out.append
+        A('extern "C" __global__ void __launch_bounds__(256) fused_boundary(const Params p, int rec0) {')
+        A("    __shared__ float red_scratch[4 * 256 / 64 + 4];")
+        A("    Acc A;")
+        for ri in range(len(g.reductions)):
+            A(f"    A.r{ri}.init();")
+        for i in range(ndet):
+            A(f"    A.dflag{i} = 0.0f;")
+        for i in range(len(g.dscalars)):
+            A(f"    A.sc{i} = (float)p.scal{i}[0];")
+        A("    const i64 t = (i64)blockIdx.x * 256 + threadIdx.x;")
+        A("    int z = -1, y = 0, x = 0;")
+        off = 0
+        for (z0, z1, y0, y1, x0, x1) in boxes:
+            ny, nx = y1 - y0, x1 - x0
+            size = (z1 - z0) * ny * nx
+            A(f"    if (z < 0 && t >= {off} && t < {off + size}) {{ const i64 q = t - {off}; "
+              f"x = {x0} + (int)(q % {nx}); y = {y0} + (int)((q / {nx}) % {ny}); z = {z0} + (int)(q / {nx * ny}); }}")
+            off += size
+        qa = "".join(", nullptr, nullptr, nullptr" for _ in self.queue_sources())
+        A("    if (z >= 0) {")
+        A("        const IDX vox = ((IDX)z * N1 + y) * N2 + x;")
+        A(f"        ev_gather(p, A, 0, 0, z, y, x, 0, 0, 0, 0, vox{qa});")
+        A("    }")
+        for ri, nid in enumerate(g.reductions):
+            op = self.graph.node(nid).op_name
+            fn = {"variance": "block_var", "min_max": "block_mm"}.get(op, "block_sum")
+            A(f"    {fn}(A.r{ri}, red_scratch, p.red{ri} + (i64)(rec0 + blockIdx.x) * 4);")
+        for i in range(ndet):
+            A(f"    {{ RMinMax fm; fm.lo = INFINITY; fm.hi = A.dflag{i} > 0.0f ? 1.0f : -INFINITY;"
+              f" block_mm(fm, red_scratch, p.dflag{i} + (i64)(rec0 + blockIdx.x) * 4); }}")
+        A("}")
+        return "\n".join(out)
+
     def _walk(self, e: "_Emit") -> None:
         pos = e.origin()
         for nid in self.group.outputs:
@@ -873,7 +938,7 @@ class KernelGen:
         tot = 0
         for i in self.lds_sources():
             L_ = self.src_layout[i]
-            tot += L_["D"] * L_["C"] * (s.ty + 2 * L_["H"]) * (s.tx + 2 * L_["H"]) * 4
+            tot += L_["D"] * (L_["C"] + 1) * (s.ty + 2 * L_["H"]) * (s.tx + 2 * L_["H"]) * 4
         return tot + 4 * (s.threads // 64) * 4 + 16
 
     # -- per-voxel code ------------------------------------------------------------------------
@@ -958,13 +1023,16 @@ class KernelGen:
         lines.append("struct Acc {\n" + "\n".join(acc) + "\n};\n")
         for i in self.lds_sources():
             L_ = self.src_layout[i]
-            lines.append(f"__shared__ float lds_s{i}[{L_['D']}][{L_['C']}][TY + {2 * L_['H']}][TX + {2 * L_['H']}];")
+            # one spare row per slot absorbs the stores of unused halo entries
+            lines.append(f"__shared__ float lds_s{i}[{L_['D']}][{L_['C'] + 1}][TY + {2 * L_['H']}][TX + {2 * L_['H']}];")
         fns = []
         for m in self._variants():
             name, fn = self._emit_variant(m)
             fns.append((m, name))
             lines.append(fn)
         lines.append(self._kernel(fns, ndet))
+        if self.use_lds:
+            lines.append(self._boundary_kernel(ndet))
         return "\n".join(lines)
 
     # -- kernel skeleton ---------------------------------------------------------------------
@@ -995,6 +1063,17 @@ class KernelGen:
         # partial tiles (last tile of an axis not divisible by the tile) use the gather
         # variant: one-sided edge formulas there reach inward past the tile start
         A("    const bool partial = (x0 + TX > N2) || (y0 + TY > N1);")
+        if self.use_lds:
+            # partial tiles are evaluated by the boundary kernel; write empty records
+            A("    if (partial) {")
+            for ri, nid in enumerate(g.reductions):
+                A(f"        if (tid == 0) {{ double* r_ = p.red{ri} + (i64)blockIdx.x * 4; r_[0] = 0.0; r_[1] = 0.0; "
+                  f"r_[2] = {'INFINITY' if self.graph.node(nid).op_name == 'min_max' else '0.0'}; r_[3] = 0.0; "
+                  f"{'r_[0] = INFINITY; r_[1] = -INFINITY;' if self.graph.node(nid).op_name == 'min_max' else ''} }}")
+            for i in range(ndet):
+                A(f"        if (tid == 0) {{ double* r_ = p.dflag{i} + (i64)blockIdx.x * 4; r_[0] = INFINITY; r_[1] = -INFINITY; r_[2] = 0.0; r_[3] = 0.0; }}")
+            A("        return;")
+            A("    }")
         names = {m: n for m, n in fns}
         qs = self.queue_sources()
         lds = self.lds_sources()
@@ -1055,7 +1134,8 @@ class KernelGen:
                 A(f"        else {{ const int e3 = e - {top + mid}; r = TY + {H} + e3 / {W * C};"
                   f" const int q = e3 - (r - TY - {H}) * {W * C}; col = q / {C}; ch = q - col * {C}; }}")
                 A(f"        hg{i}[j] = (wrapm(y0 - {H} + r, N1) * N2 + wrapm(x0 - {H} + col, N2)) * {C} + ch;")
-                A(f"        hl{i}[j] = e < {HT} ? (ch * {RH} + r) * {W} + col : -1;")
+                # out-of-range entries write a spare word at the end of the slot array
+                A(f"        hl{i}[j] = e < {HT} ? (ch * {RH} + r) * {W} + col : {C * RH * W};")
                 A("    }")
             for buf in ("A", "B"):
                 A(f"    float {buf}o{i}[VPT][{C}];")
@@ -1074,7 +1154,7 @@ class KernelGen:
         A(f"{indent}    for (int c = 0; c < {C}; ++c) {buf}o{i}[k][c] = p.src{i}[pb + og{i}[k] + c];")
         if KH:
             A(f"{indent}  #pragma unroll")
-            A(f"{indent}  for (int j = 0; j < {KH}; ++j) if (hl{i}[j] >= 0) {buf}h{i}[j] = p.src{i}[pb + hg{i}[j]];")
+            A(f"{indent}  for (int j = 0; j < {KH}; ++j) {buf}h{i}[j] = p.src{i}[pb + hg{i}[j]];")
         A(f"{indent}}}")
 
     def _store(self, A, i: int, buf: str, slot: str, indent: str = "    ") -> None:
@@ -1090,7 +1170,7 @@ class KernelGen:
         A(f"{indent}    for (int c = 0; c < {C}; ++c) ls[(c * {RH} + ly0 + k * RP + {H}) * {W} + lx + {H}] = {buf}o{i}[k][c];")
         if KH:
             A(f"{indent}  #pragma unroll")
-            A(f"{indent}  for (int j = 0; j < {KH}; ++j) if (hl{i}[j] >= 0) ls[hl{i}[j]] = {buf}h{i}[j];")
+            A(f"{indent}  for (int j = 0; j < {KH}; ++j) ls[hl{i}[j]] = {buf}h{i}[j];")
         A(f"{indent}}}")
 
     def _lead(self, i: int) -> int:
@@ -1127,7 +1207,8 @@ class KernelGen:
         qa = "".join(f", qm{i}[k], q0{i}[k], {buf}o{i}[k]" for i in self.queue_sources())
         call = f"(p, A, s, {sq}, {z}, y, x, ly, lx, y0, x0, vox{qa});"
         if ("lds", False, False) in names:
-            A(f"{ind}      if (ez || partial) ev_gather{call}")
+            # z-edge planes belong to the boundary kernel
+            A(f"{ind}      if (ez) {{ }}")
             A(f"{ind}      else if (!ex && !ey) ev_lds00{call}")
             A(f"{ind}      else if (ex && !ey) ev_lds10{call}")
             A(f"{ind}      else if (!ex) ev_lds01{call}")
@@ -1147,10 +1228,11 @@ class KernelGen:
                 A(f"{ind}  for (int k = 0; k < VPT; ++k)")
                 A(f"{ind}    #pragma unroll")
                 A(f"{ind}    for (int c = 0; c < {C}; ++c) {{ qm{i}[k][c] = q0{i}[k][c]; q0{i}[k][c] = {buf}o{i}[k][c]; }}")
-            A(f"{ind}  if ({z} + 2 < ze) {{")
+            # always fetch (clamped to the last plane this block needs, an L2 hit) so that
+            # every path issues the same loads and the compiler can wait on counts, not zero
             for i in range(len(self.src_layout)):
-                self._fetch(A, i, buf, f"{z} + 2 + {self._lead(i)}", ind + "    ")
-            A(f"{ind}  }}")
+                ld = self._lead(i)
+                self._fetch(A, i, buf, f"min({z} + 2 + {ld}, ze - 1 + {ld})", ind + "  ")
         A(f"{ind}}}")
 
 
@@ -1205,10 +1287,17 @@ class CompiledGroup:
     gen: KernelGen
     source: str
     module: Optional[L.Module] = None
+    bmodule: Optional[L.Module] = None
 
     @property
     def group(self) -> Group:
         return self.gen.group
+
+    def boundary_module(self) -> L.Module:
+        if self.bmodule is None:
+            code, _ = compile_source(self.source)
+            self.bmodule = L.Module(code, "fused_boundary")
+        return self.bmodule
 
     def nblocks(self) -> Tuple[int, int, int, int]:
         s = self.gen.sched
@@ -1331,9 +1420,10 @@ class FusedExecutor:
             dt = cg.gen.out_dtypes.get(o, torch.float32)
             outs.append(torch.empty(shape, dtype=dt, device=dev))
         tx, ty, zcn, nb = cg.nblocks()
-        reds = [torch.empty(nb * 4, dtype=torch.float64, device=dev) for _ in g.reductions]
+        nrec = nb + cg.gen.boundary_blocks()
+        reds = [torch.empty(nrec * 4, dtype=torch.float64, device=dev) for _ in g.reductions]
         ndet = len(cg.gen._det_inputs())
-        dflags = [torch.empty(nb * 4, dtype=torch.float64, device=dev) for _ in range(ndet)]
+        dflags = [torch.empty(nrec * 4, dtype=torch.float64, device=dev) for _ in range(ndet)]
         scal = []
         for d in g.dscalars:
             t = results[d]
@@ -1349,7 +1439,7 @@ class FusedExecutor:
         ln = self.prepare_launch(cg, graph, results)
         stream = L.stream_ptr(dev)
         ln.launch(stream)
-        nb = ln.nblocks
+        nb = ln.nblocks + cg.gen.boundary_blocks()
         if ln.dflags:
             flag = torch.empty(2, dtype=torch.float64, device=dev)
             L.check(L.lib().sp_reduce_combine(L.REDUCE["any_small_diag"], ln.dflags[0].data_ptr(), nb,
@@ -1384,22 +1474,25 @@ class Launch:
         return pack_params(self.cg, self.srcs, self.outs, self.reds, self.dflags, self.scal, det_reg)
 
     def launch(self, stream: Optional[int] = None, det_reg: float = 0.0) -> None:
-        self.cg.module.launch((self.nblocks, 1, 1), (self.cg.gen.sched.threads, 1, 1),
-                              self.args(det_reg), 0, stream if stream is not None else L.stream_ptr())
+        st = stream if stream is not None else L.stream_ptr()
+        a = self.args(det_reg)
+        self.cg.module.launch((self.nblocks, 1, 1), (self.cg.gen.sched.threads, 1, 1), a, 0, st)
+        nbb = self.cg.gen.boundary_blocks()
+        if nbb:
+            self.cg.boundary_module().launch((nbb, 1, 1), (256, 1, 1),
+                                             a + struct.pack("<i", self.nblocks), 0, st)
 
     def time_ms(self, reps: int = 20, warmup: int = 3, rounds: int = 3) -> float:
         """Min over ``rounds`` of the mean kernel time, HIP events on the launch stream."""
         s = L.stream_ptr()
-        a = self.args()
-        grid, block = (self.nblocks, 1, 1), (self.cg.gen.sched.threads, 1, 1)
         for _ in range(warmup):
-            self.cg.module.launch(grid, block, a, 0, s)
+            self.launch(s)
         best = float("inf")
         for _ in range(rounds):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
             for _ in range(reps):
-                self.cg.module.launch(grid, block, a, 0, s)
+                self.launch(s)
             e1.record()
             e1.synchronize()
             best = min(best, e0.elapsed_time(e1) / reps)

@@ -48,7 +48,7 @@ def test_c2_graph(shape):
     run_and_check(C2_EXPR, {"f": f, "u": u}, reductions=("sum", "mean", "variance", "norm_l2"))
 
 
-@pytest.mark.parametrize("sched", [Schedule(32, 4, 16, 256), Schedule(64, 8, 64, 256),
+@pytest.mark.parametrize("sched", [Schedule(32, 8, 16, 256), Schedule(64, 8, 64, 256),
                                    Schedule(128, 4, 32, 256), Schedule(64, 8, 8, 512)])
 def test_c2_schedules(sched):
     shape = (40, 45, 150)
