@@ -84,7 +84,14 @@ class Schedule:
 DEFAULT_SCHEDULE = Schedule()
 # LDS per block kept within 64 KiB so that at least two blocks share a CU
 LDS_LIMIT = 64 * 1024
-FALLBACK_SCHEDULES = [Schedule(64, 4, 64, 256), Schedule(32, 8, 64, 256), Schedule(64, 8, 64, 512)]
+FALLBACK_SCHEDULES = [Schedule(64, 4, 64, 256), Schedule(32, 8, 64, 256), Schedule(32, 4, 64, 128),
+                      Schedule(16, 8, 64, 128), Schedule(16, 4, 64, 64)]
+# Edge handling of the main kernel.  "band": only voxels at least the edge reach away from
+# every face run in the LDS kernel (one code variant, low register pressure) and the thin
+# face band goes to the gather kernel.  "variants": tiles touching an x/y face run
+# edge-specialised LDS variants.  "auto" picks "band" when the band is a small share.
+EDGE_MODE = os.environ.get("STENPY_EDGE_MODE", "auto")
+BAND_MAX_FRACTION = 0.08
 
 
 # ------------------------------------------------------------------------------ node typing
@@ -301,6 +308,7 @@ def plan_graph(graph: Graph, values: Dict[str, Any], wanted: Sequence[str]) -> T
             groups[key] = g
             steps.append(Step("group", group=g, level=inf.level))
         g.nodes.append(nid)
+        g._info = info
         if node.op_name in REDUCTION_OPS:
             g.reductions.append(nid)
     # sources, scalars, outputs, reach
@@ -804,7 +812,8 @@ class KernelGen:
     """
 
     def __init__(self, graph: Graph, group: Group, info: Dict[str, Info], sched: Schedule,
-                 out_dtypes: Dict[str, torch.dtype], src_meta: Dict[str, Tuple[int, ...]]):
+                 out_dtypes: Dict[str, torch.dtype], src_meta: Dict[str, Tuple[int, ...]],
+                 edge_mode: Optional[str] = None):
         self.graph, self.group, self.info, self.sched = graph, group, info, sched
         self.grid = group.grid
         self.out_dtypes = out_dtypes
@@ -820,7 +829,33 @@ class KernelGen:
         self.v = sched.ty // self.rp                 # voxels per thread per plane
         self.src_layout = [{"C": info[s].ch, "H": reach[s][2], "D": 2 * self.hz + 2,
                             "mode": "ring", "lds": True} for s in group.sources]
+        self.edge_mode = self._pick_edge_mode(edge_mode or EDGE_MODE)
         self._choose_modes()
+
+    def _pick_edge_mode(self, mode: str) -> str:
+        if mode in ("band", "variants"):
+            return mode
+        n0, n1, n2 = self.grid
+        r = self.redge
+        if n1 <= 2 * r or n2 <= 2 * r:
+            return "variants"
+        frac = 1.0 - (n1 - 2 * r) * (n2 - 2 * r) / float(n1 * n2)
+        return "band" if frac <= BAND_MAX_FRACTION else "variants"
+
+    def interior_bounds(self) -> Tuple[int, int, int, int, int, int]:
+        """Box (z0, z1, y0, y1, x0, x1) of voxels the main kernel evaluates."""
+        n0, n1, n2 = self.grid
+        r, s = self.redge, self.sched
+        xp = (n2 // s.tx) * s.tx          # start of a partial x tile (== n2 if none)
+        yp = (n1 // s.ty) * s.ty
+        zlo = min(r, n0)
+        zhi = max(n0 - r, zlo)
+        if self.edge_mode == "band":
+            ylo, xlo = min(r, n1), min(r, n2)
+            yhi, xhi = max(min(yp, n1 - r), ylo), max(min(xp, n2 - r), xlo)
+        else:
+            ylo, yhi, xlo, xhi = 0, yp, 0, xp
+        return zlo, zhi, ylo, yhi, xlo, xhi
 
     # -- source modes ----------------------------------------------------------------------
     def _choose_modes(self) -> None:
@@ -850,7 +885,10 @@ class KernelGen:
     def _variants(self):
         v = [("gather",)]
         if self.use_lds:
-            v += [("lds", ex, ey) for ex in (False, True) for ey in (False, True)]
+            if self.edge_mode == "band":
+                v.append(("lds", False, False))
+            else:
+                v += [("lds", ex, ey) for ex in (False, True) for ey in (False, True)]
         return v
 
     # -- boundary kernel -------------------------------------------------------------------------
@@ -859,21 +897,10 @@ class KernelGen:
         if not self.use_lds:
             return []
         n0, n1, n2 = self.grid
-        r = self.redge
-        s = self.sched
-        xp = (n2 // s.tx) * s.tx          # start of a partial x tile (== n2 if none)
-        yp = (n1 // s.ty) * s.ty
-        boxes = []
-        zlo, zhi = min(r, n0), max(n0 - r, min(r, n0))
-        if zlo > 0:
-            boxes.append((0, zlo, 0, n1, 0, n2))
-        if zhi < n0:
-            boxes.append((zhi, n0, 0, n1, 0, n2))
-        if zhi > zlo:
-            if xp < n2:
-                boxes.append((zlo, zhi, 0, n1, xp, n2))
-            if yp < n1:
-                boxes.append((zlo, zhi, yp, n1, 0, xp))
+        zlo, zhi, ylo, yhi, xlo, xhi = self.interior_bounds()
+        boxes = [(0, zlo, 0, n1, 0, n2), (zhi, n0, 0, n1, 0, n2),
+                 (zlo, zhi, 0, ylo, 0, n2), (zlo, zhi, yhi, n1, 0, n2),
+                 (zlo, zhi, ylo, yhi, 0, xlo), (zlo, zhi, ylo, yhi, xhi, n2)]
         return [b for b in boxes if b[1] > b[0] and b[3] > b[2] and b[5] > b[4]]
 
     def boundary_blocks(self) -> int:
@@ -1005,6 +1032,9 @@ class KernelGen:
                      f"#define TX {s.tx}\n#define TY {s.ty}\n#define ZC {s.zc}\n#define NT {s.threads}\n"
                      f"#define HZ {self.hz}\n#define REDGE {self.redge}\n#define RP {self.rp}\n"
                      f"#define VPT {self.v}\ntypedef {idx_t} IDX;\n")
+        if self.use_lds:
+            _, _, ylo, yhi, xlo, xhi = self.interior_bounds()
+            lines.append(f"#define YLO {ylo}\n#define YHI {yhi}\n#define XLO {xlo}\n#define XHI {xhi}\n")
         pf = [f"    const float* __restrict__ src{i};" for i in range(ns)]
         pf += [f"    void* out{i};" for i in range(no)]
         pf += [f"    double* red{i};" for i in range(nr)]
@@ -1206,7 +1236,10 @@ class KernelGen:
         A(f"{ind}      const IDX vox = ((IDX){z} * N1 + y) * N2 + x;")
         qa = "".join(f", qm{i}[k], q0{i}[k], {buf}o{i}[k]" for i in self.queue_sources())
         call = f"(p, A, s, {sq}, {z}, y, x, ly, lx, y0, x0, vox{qa});"
-        if ("lds", False, False) in names:
+        if self.edge_mode == "band" and ("lds", False, False) in names:
+            # z-edge planes and the x/y face band belong to the boundary kernel
+            A(f"{ind}      if (!ez && y >= YLO && y < YHI && x >= XLO && x < XHI) ev_lds00{call}")
+        elif ("lds", False, False) in names:
             # z-edge planes belong to the boundary kernel
             A(f"{ind}      if (ez) {{ }}")
             A(f"{ind}      else if (!ex && !ey) ev_lds00{call}")
@@ -1319,9 +1352,10 @@ def pack_params(cg: CompiledGroup, srcs: Sequence[torch.Tensor], outs: Sequence[
 class FusedExecutor:
     """Runs graphs through fused plans; caches plans by (graph structure, shapes, schedule)."""
 
-    def __init__(self, device: torch.device, schedule_fn=None):
+    def __init__(self, device: torch.device, schedule_fn=None, edge_mode: Optional[str] = None):
         self.device = device
         self.schedule_fn = schedule_fn
+        self.edge_mode = edge_mode
         self._plans: Dict[str, Tuple[List[Step], Dict[str, Info], Dict[int, CompiledGroup]]] = {}
         self.last_timing: Dict[str, float] = {}
 
@@ -1338,7 +1372,7 @@ class FusedExecutor:
                               for k, v in values.items()))
         key = graph.canonical_key((shapes, tuple(sorted(wanted)), str(out_dtype),
                                    schedule.as_tuple() if schedule else None,
-                                   id(self.schedule_fn)))
+                                   id(self.schedule_fn), self.edge_mode or EDGE_MODE))
         plan = self._plans.get(key)
         if plan is not None:
             return plan
@@ -1357,10 +1391,10 @@ class FusedExecutor:
             for nid in g.outputs:
                 if graph.node(nid).op_name == "cast_bf16":
                     od[nid] = torch.bfloat16
-            gen = KernelGen(graph, g, info, sched, od, {})
+            gen = KernelGen(graph, g, info, sched, od, {}, self.edge_mode)
             if gen.lds_bytes() > LDS_LIMIT and schedule is None:
                 for alt in FALLBACK_SCHEDULES:
-                    gen = KernelGen(graph, g, info, alt, od, {})
+                    gen = KernelGen(graph, g, info, alt, od, {}, self.edge_mode)
                     if gen.lds_bytes() <= LDS_LIMIT:
                         break
             if gen.lds_bytes() > LDS_LIMIT:

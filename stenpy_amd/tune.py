@@ -25,11 +25,14 @@ from .fused import Schedule
 
 _DB_LOCK = threading.Lock()
 
+# threads must cover whole tile rows: NT % TX == 0 and TY a multiple of NT / TX
 CANDIDATES: List[Schedule] = [
     Schedule(tx, ty, zc, nt)
     for tx in (32, 64, 128) for ty in (4, 8, 16) for zc in (16, 32, 64, 128) for nt in (256, 512)
-    if tx * ty >= nt // 2 and not (tx == 32 and ty == 4)
+    if nt % tx == 0 and ty % (nt // tx) == 0
 ]
+# a smaller grid of candidates for the training-data sweep
+DATA_CANDIDATES: List[Schedule] = [c for c in CANDIDATES if c.zc in (32, 64, 128)]
 
 
 # ------------------------------------------------------------------------------ database
@@ -152,7 +155,10 @@ def _compile_all(sources: List[str], workers: int) -> List[bytes]:
 def candidates_for(graph, group, info, out_dtypes, cands: Sequence[Schedule] = None):
     gens = []
     for s in (cands or CANDIDATES):
-        gen = F.KernelGen(graph, group, info, s, out_dtypes, {})
+        try:
+            gen = F.KernelGen(graph, group, info, s, out_dtypes, {})
+        except ValueError:
+            continue
         if gen.lds_bytes() <= F.LDS_LIMIT:
             gens.append(gen)
     return gens
@@ -201,3 +207,108 @@ def default_time(rows: List[Dict[str, Any]], sched: Schedule = F.DEFAULT_SCHEDUL
         if tuple(r["schedule"]) == sched.as_tuple():
             return r["ms"]
     return None
+
+
+# ------------------------------------------------------------------------------ learned model
+def random_group_rows(n_graphs: int, seed: int, runtime, cands: Sequence[Schedule] = None,
+                      shape_range: Tuple[int, int] = (64, 256), reps: int = 5, workers: int = 16,
+                      log=None) -> List[Dict[str, Any]]:
+    """Exhaustive timings of random config-3-style graphs on random grids (training data)."""
+    import random
+    from . import randgraph
+    from .compiler import compile_expression
+    from . import gen
+    rng = random.Random(seed)
+    rows: List[Dict[str, Any]] = []
+    for gi in range(n_graphs):
+        expr, nops = randgraph.random_expression(rng, rng.randint(3, 12))
+        shape = randgraph.random_shape(rng, *shape_range)
+        f = gen.generate(shape, 1000 + gi, runtime.device)
+        u = gen.generate_vector(shape, (2000 + gi, 3000 + gi, 4000 + gi), runtime.device)
+        g, sink, *_ = compile_expression(expr, boundary=rng.choice(["periodic", "neumann"]),
+                                         field_map={"f": f, "u": u})
+        try:
+            r = autotune(g, runtime, [sink], cands=cands, reps=reps, workers=workers, update_db=False)
+        except Exception as exc:  # a graph the planner cannot fuse is skipped
+            if log:
+                log(f"graph {gi} skipped: {exc}")
+            continue
+        for row in r:
+            row["graph"] = gi
+            row["expr"] = expr
+        rows.extend(r)
+        if log:
+            log(f"graph {gi}: {expr[:60]} {shape} -> {len(r)} timings")
+    return rows
+
+
+class ScheduleModel:
+    """MLP over (group, schedule) features predicting log(ns per voxel); picks the argmin."""
+
+    def __init__(self, mlp=None, cands: Sequence[Schedule] = None):
+        self.mlp = mlp
+        self.cands = list(cands or CANDIDATES)
+
+    def fit(self, rows: List[Dict[str, Any]], device="cuda", epochs: int = 150, seed: int = 0):
+        from .mlp import ScheduleMLP
+        X = np.asarray([r["features"] for r in rows], dtype=np.float32)
+        y = np.log(np.asarray([r["ns_per_voxel"] for r in rows], dtype=np.float32))
+        self.mlp = ScheduleMLP(X.shape[1], 256, 4, device, seed)
+        return self.mlp.fit(X, y, epochs=epochs, batch=256, lr=2e-3, seed=seed)
+
+    def predict_rows(self, rows: List[Dict[str, Any]]) -> np.ndarray:
+        X = np.asarray([r["features"] for r in rows], dtype=np.float32)
+        return self.mlp.predict(X)
+
+    def best(self, group, graph) -> Optional[Schedule]:
+        info = getattr(group, "_info", None)
+        try:
+            base = F.KernelGen(graph, group, info, F.DEFAULT_SCHEDULE, {}, {})
+        except Exception:
+            return None
+        stats = group_stats(base)
+        feats, scheds = [], []
+        for s in self.cands:
+            try:
+                gen = F.KernelGen(graph, group, info, s, {}, {})
+            except ValueError:
+                continue
+            if gen.lds_bytes() > F.LDS_LIMIT:
+                continue
+            feats.append(features(stats, group.grid, s, gen.lds_bytes()))
+            scheds.append(s)
+        if not scheds:
+            return None
+        pred = self.mlp.predict(np.asarray(feats, dtype=np.float32))
+        return scheds[int(np.argmin(pred))]
+
+    def save(self, path) -> None:
+        torch.save({"mlp": self.mlp.state_dict(), "cands": [c.as_tuple() for c in self.cands]}, path)
+
+    @classmethod
+    def load(cls, path, device="cuda") -> "ScheduleModel":
+        from .mlp import ScheduleMLP
+        st = torch.load(path, map_location="cpu", weights_only=True)
+        dims = st["mlp"]["dims"]
+        mlp = ScheduleMLP(dims[0], dims[1], len(dims) - 1, device)
+        mlp.load_state_dict(st["mlp"])
+        return cls(mlp, [Schedule(*c) for c in st["cands"]])
+
+
+def evaluate_choice(rows: List[Dict[str, Any]], model: ScheduleModel) -> Dict[str, float]:
+    """Per group: time of the model's pick vs the exhaustive best and vs the default."""
+    by: Dict[Tuple, List[Dict[str, Any]]] = {}
+    for r in rows:
+        by.setdefault((r.get("graph"), r["sig"]), []).append(r)
+    regrets, gains, hits = [], [], 0
+    for key, rs in by.items():
+        pred = model.predict_rows(rs)
+        pick = rs[int(np.argmin(pred))]["ms"]
+        best = min(r["ms"] for r in rs)
+        dflt = default_time(rs)
+        regrets.append(pick / best - 1.0)
+        if dflt is not None:
+            gains.append(dflt / pick)
+        hits += pick == best
+    return {"groups": len(by), "mean_regret": float(np.mean(regrets)), "max_regret": float(np.max(regrets)),
+            "top1": hits / max(len(by), 1), "geomean_gain_vs_default": float(np.exp(np.mean(np.log(gains)))) if gains else None}

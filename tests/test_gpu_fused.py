@@ -16,10 +16,11 @@ C2_EXPR = "divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))"
 
 
 def run_and_check(expr, fields, boundary="periodic", dx=1.0, reductions=(), schedule=None,
-                  out_dtype=None, slack=None):
+                  out_dtype=None, slack=None, edge_mode=None):
     g, sink, _, _, _ = sp.compile_expression(expr, dx=dx, boundary=boundary, field_map=fields)
     reds = [g.add(r, (sink,), {}) for r in reductions]
     rt = sp.Runtime(device="cuda", fusion_enabled=True)
+    rt.fused.edge_mode = edge_mode
     res = rt.fused.run(g, rt, wanted=[sink] + reds, out_dtype=out_dtype, schedule=schedule)
     o64 = graph_eval.evaluate(g, torch.float64)
     o32 = graph_eval.evaluate(g, torch.float32)
@@ -41,11 +42,13 @@ def test_c1_expressions():
         run_and_check(expr, {"f": f})
 
 
+@pytest.mark.parametrize("edge_mode", ["band", "variants"])
 @pytest.mark.parametrize("shape", [(32, 32, 32), (21, 37, 70), (9, 8, 130)])
-def test_c2_graph(shape):
+def test_c2_graph(shape, edge_mode):
     f = field(shape, 2)
     u = vfield(shape)
-    run_and_check(C2_EXPR, {"f": f, "u": u}, reductions=("sum", "mean", "variance", "norm_l2"))
+    run_and_check(C2_EXPR, {"f": f, "u": u}, reductions=("sum", "mean", "variance", "norm_l2"),
+                  edge_mode=edge_mode)
 
 
 @pytest.mark.parametrize("sched", [Schedule(32, 8, 16, 256), Schedule(64, 8, 64, 256),
@@ -62,23 +65,25 @@ def test_c2_bf16_output():
                   reductions=("sum", "mean", "variance", "norm_l2"), out_dtype=torch.bfloat16)
 
 
+@pytest.mark.parametrize("edge_mode", ["band", "variants"])
 @pytest.mark.parametrize("bc", ["periodic", "neumann", "dirichlet", "reflect"])
-def test_stencil_compositions(bc):
+def test_stencil_compositions(bc, edge_mode):
     shape = (18, 22, 70)
     f = field(shape, 30)
     u = vfield(shape)
     for expr in ["laplacian(laplacian(f))", "divergence(curl(u)) + laplacian(exp(f))",
                  "gradient(f) * 2 + curl(u)", "laplacian(f) * tanh(f) - clamp(f, lo=-0.1, hi=0.2)",
                  "divergence(u) + sqrt(f**2 + 1) + log(f**2)"]:
-        run_and_check(expr, {"f": f, "u": u}, boundary=bc)
+        run_and_check(expr, {"f": f, "u": u}, boundary=bc, edge_mode=edge_mode)
 
 
-def test_config5_tensor_ops():
+@pytest.mark.parametrize("edge_mode", ["band", "variants"])
+def test_config5_tensor_ops(edge_mode):
     shape = (20, 24, 30)
     f = field(shape, 8)
     for expr in ["eigenvalues(hessian(f))", "determinant(hessian(f))", "trace(hessian(f))",
                  "deviatoric(hessian(f))", "mean_curvature(f)", "jacobian(curl(u))"]:
-        run_and_check(expr, {"f": f, "u": vfield(shape)}, boundary="periodic")
+        run_and_check(expr, {"f": f, "u": vfield(shape)}, boundary="periodic", edge_mode=edge_mode)
 
 
 def test_scalar_from_reduction_feeds_next_group():
