@@ -79,6 +79,52 @@ def cpu_baseline(n: int, planes: int):
     return vox / ts[1], threads, f"{planes}x{n}x{n} slab, FP64 torch CPU oracle, median of 3"
 
 
+def copy_bytes(n: int) -> int:
+    return 4 * n ** 3
+
+
+def profile_pass(args) -> int:
+    """Fused C2 launches only (plus one calibration copy): what rocprofv3 measures."""
+    import torch
+    import stenpy_amd as sp
+    from stenpy_amd import _lib as L, gen, tune
+    dev = torch.device("cuda:0")
+    n = args.n
+    f = gen.generate((n, n, n), 2, dev)
+    u = gen.generate_vector((n, n, n), (3, 4, 5), dev)
+    g, sink, reds = build_graph(f, u)
+    rt = sp.Runtime(device=dev, fusion_enabled=True, schedule_fn=tune.schedule_fn(dev))
+    res = rt.fused.run(g, rt, wanted=[sink] + reds)
+    cg, ln = group_launch(rt, g, res, sink)
+    for _ in range(args.steps):
+        ln.launch()
+    out = torch.empty_like(f)
+    for _ in range(3):
+        L.unary("neg", f, out)          # streaming copy: reads and writes copy_bytes(n)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile_pass": True, "schedule": list(cg.gen.sched.as_tuple()),
+                      "launches": args.steps, "grid": ln.nblocks + cg.gen.boundary_blocks(),
+                      "copy_bytes": copy_bytes(n)}), flush=True)
+    return 0
+
+
+def offline_reports(args) -> int:
+    from stenpy_amd import profiling
+    if args.pmc_traffic:
+        t = profiling.traffic(args.pmc_traffic[0], args.pmc_traffic[1], kernel="fused_kernel",
+                              copy_kernel="k_unary_contig", copy_bytes=copy_bytes(args.n))
+        t["algorithmic_bytes"] = 20 * args.n ** 3
+        t["n"] = args.n
+        text = json.dumps(t, indent=1) + "\n"
+    else:
+        from stenpy_amd import profiling as P
+        text = P.stats_markdown(args.stats_md, args.cmd)
+    if args.out:
+        Path(args.out).write_text(text)
+    print(text)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,12 +133,25 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-planes", type=int, default=32)
-    ap.add_argument("--traffic", type=str, default=None,
+    ap.add_argument("--traffic", type=str, default=str(ROOT / "profiles" / "traffic_c2_fused_512.json"),
                     help="JSON file with measured HBM bytes per launch of the fused kernel")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="only launch the fused C2 kernel --steps times plus a streaming copy of known "
+                         "size (the target of rocprofv3 --pmc / --kernel-trace runs)")
+    ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None,
+                    help="turn two rocprofv3 --pmc CSV outputs into the traffic JSON (no GPU)")
+    ap.add_argument("--stats-md", type=str, default=None, metavar="DIR",
+                    help="render a rocprofv3 --stats CSV output as markdown (no GPU)")
+    ap.add_argument("--cmd", type=str, default="", help="command line quoted in --stats-md output")
+    ap.add_argument("--out", type=str, default=None)
     args = ap.parse_args()
+    if args.pmc_traffic or args.stats_md:
+        return offline_reports(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs on one GPU; --gpus must be 1"}))
         return 2
+    if args.profile_pass:
+        return profile_pass(args)
 
     import torch
     import stenpy_amd as sp
@@ -130,7 +189,9 @@ def main() -> int:
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
     traffic = None
     if args.traffic and Path(args.traffic).exists():
-        traffic = json.loads(Path(args.traffic).read_text()).get("bytes_per_launch")
+        tj = json.loads(Path(args.traffic).read_text())
+        if tj.get("n") == n and tuple(tj.get("schedule", ())) in ((), tuple(cg.gen.sched.as_tuple())):
+            traffic = round(tj.get("bytes_per_launch"))
 
     line = {
         "metric": "Gvoxel/s + % HBM peak: fused/autotuned vs plain per-op HIP, 512^3 graph",
@@ -152,7 +213,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel_ms": round(k_ms, 4),
-                     "kernels": "fused_kernel + fused_boundary (z-edge planes, partial tiles)",
+                     "kernels": "fused_kernel (LDS z-march over full tiles + tail blocks for the face band)",
                      "algorithmic_bytes_per_launch": alg_bytes},
     }
     if not args.no_cpu:

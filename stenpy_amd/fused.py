@@ -345,6 +345,21 @@ def plan_graph(graph: Graph, values: Dict[str, Any], wanted: Sequence[str]) -> T
 
 
 # ------------------------------------------------------------------------------ codegen
+def _ct_index(fn: str, c: int, n: int) -> int:
+    """Compile-time counterparts of the device coordinate helpers wrapc/clampc/reflc."""
+    if fn == "wrapc":
+        return c + n if c < 0 else (c - n if c >= n else c)
+    if fn == "clampc":
+        return 0 if c < 0 else (n - 1 if c >= n else c)
+    if fn == "reflc":
+        if c < 0:
+            return -c if -c < n - 1 else n - 1
+        if c >= n:
+            return 2 * (n - 1) - c if 2 * (n - 1) - c > 0 else 0
+        return c
+    raise KeyError(fn)
+
+
 def _flt(x: float) -> str:
     f = float(np.float32(x))
     if math.isnan(f):
@@ -368,9 +383,11 @@ class _NeedRing(Exception):
 class _Emit:
     """Per-variant expression emitter with (node, position) memoisation.
 
-    ``mode`` is ("lds", ex, ey) or ("gather",).  A position is a 3-tuple of axis
-    coordinates, each ("I", k) -- the voxel's own coordinate plus a compile-time offset --
-    or ("E", expr) -- a runtime expression for an absolute coordinate.
+    ``mode`` is ("lds", ex, ey), ("xface", side, d) or ("gather",).  A position is a 3-tuple
+    of axis coordinates, each ("I", k) -- the voxel's own coordinate plus a compile-time
+    offset --, ("E", expr) -- a runtime expression for an absolute coordinate -- or ("A", c),
+    an absolute coordinate known at compile time (the x coordinate in the x-face variants,
+    which evaluate the voxels d cells from the low or high x face of a full tile).
     """
 
     AX_VAR = ("z", "y", "x")
@@ -395,23 +412,31 @@ class _Emit:
     def edge_axis(self, a: int) -> bool:
         if self.mode[0] == "gather":
             return True
-        if a == 0:
+        if self.mode[0] == "xface" or a == 0:
             return False
         return bool(self.mode[1] if a == 2 else self.mode[2])
 
+    def face_x(self) -> int:
+        _, side, d = self.mode
+        return d if side == "lo" else self.g.grid[2] - 1 - d
+
     def origin(self) -> Tuple:
+        if self.mode[0] == "xface":
+            return (("I", 0), ("I", 0), ("A", self.face_x()))
         return tuple(("E", self.AX_VAR[a]) if self.edge_axis(a) else ("I", 0) for a in range(3))
 
     def cexpr(self, c, a: int) -> str:
         if c[0] == "I":
             k = c[1]
             return f"({self.AX_VAR[a]}{k:+d})" if k else self.AX_VAR[a]
+        if c[0] == "A":
+            return str(c[1])
         return c[1]
 
     def shift(self, pos, a: int, d: int):
         c = pos[a]
-        if c[0] == "I":
-            nc = ("I", c[1] + d)
+        if c[0] in ("I", "A"):
+            nc = (c[0], c[1] + d)
         else:
             e = f"({c[1]}{d:+d})"
             nc = ("E", self._cvar(e))
@@ -428,23 +453,33 @@ class _Emit:
         c = pos[a]
         if c[0] == "I":
             return pos
-        nc = ("E", self._cvar(f"{fn}({c[1]}, {self.AX_N[a]})"))
+        if c[0] == "A":
+            nc = ("A", _ct_index(fn, c[1], self.g.grid[a]))
+        else:
+            nc = ("E", self._cvar(f"{fn}({c[1]}, {self.AX_N[a]})"))
         return tuple(nc if i == a else pos[i] for i in range(3))
 
     def inside(self, pos, a: int) -> str:
         c = pos[a]
         if c[0] == "I":
             return "true"
+        if c[0] == "A":
+            return "true" if 0 <= c[1] < self.g.grid[a] else "false"
         return f"inside({c[1]}, {self.AX_N[a]})"
 
     def at(self, pos, a: int, which: str) -> str:
         c = pos[a]
         if c[0] == "I":
             return "false"
+        if c[0] == "A":
+            hit = c[1] == 0 if which == "lo" else c[1] == self.g.grid[a] - 1
+            return "true" if hit else "false"
         return f"({c[1]} == 0)" if which == "lo" else f"({c[1]} == {self.AX_N[a]} - 1)"
 
     # -- taps --------------------------------------------------------------------------
     def _is_origin(self, c, a: int) -> bool:
+        if self.mode[0] == "xface" and a == 2:
+            return c == ("A", self.face_x())
         return c == ("I", 0) or (c[0] == "E" and c[1] == self.AX_VAR[a])
 
     def tap(self, si: int, ch: int, pos) -> str:
@@ -481,6 +516,17 @@ class _Emit:
             c = pos[a]
             if c[0] == "I":
                 rows.append(f"({lvar}+{H + c[1]})")
+            elif c[0] == "A":
+                # the x-face tile starts at 0 (low face) or N2 - TX (high face); same mapping
+                # as lmap, evaluated here
+                tx, n2 = g.sched.tx, g.grid[2]
+                l = c[1] - (0 if self.mode[1] == "lo" else n2 - tx)
+                if l >= tx + H:
+                    l -= n2
+                if l < -H:
+                    l += n2
+                l = min(max(l, -H), tx + H - 1)
+                rows.append(str(l + H))
             else:
                 rows.append(self._cvar(f"lmap({c[1]}, {t0}, {T}, {H}, {N})"))
         v = self.var(f"lds_s{si}[{slot}][{ch}][{rows[0]}][{rows[1]}]")
@@ -584,10 +630,24 @@ class _Emit:
         if key in self.memo:
             return self.memo[key][0]
         hs = _flt(h)
+        if pos[a][0] == "A" and pos[a][1] in (0, self.g.grid[a] - 1):
+            # compile-time face coordinate: only the one-sided formula
+            v0 = self.val(inner, pos)[ch]
+            lo = pos[a][1] == 0
+            s1, s2 = (1, 2) if lo else (-1, -2)
+            v1 = self.val(inner, self.shift(pos, a, s1))[ch]
+            if order == 2:
+                v2 = self.val(inner, self.shift(pos, a, s2))[ch]
+                e = f"tg_left2({v0}, {v1}, {v2}, {hs})" if lo else f"tg_right2({v2}, {v1}, {v0}, {hs})"
+            else:
+                e = f"tg_left1({v0}, {v1}, {hs})" if lo else f"tg_right1({v1}, {v0}, {hs})"
+            r = self.var(e)
+            self.memo[key] = [r]
+            return r
         vp = self.val(inner, self.shift(pos, a, +1))[ch]
         vm = self.val(inner, self.shift(pos, a, -1))[ch]
         mid = f"tg_mid({vp}, {vm}, {hs})"
-        if pos[a][0] == "I":
+        if pos[a][0] in ("I", "A"):
             r = self.var(mid)
         else:
             v0 = self.val(inner, pos)[ch]
@@ -797,6 +857,45 @@ def source_reach(graph: Graph, group: Group) -> Dict[str, Tuple[int, int, int]]:
     return out
 
 
+def edge_band(graph: Graph, group: Group) -> int:
+    """Smallest distance from a face at which interior formulas are exact for the group.
+
+    The LDS kernel evaluates every op with its interior formula on wrapped reads (halo and
+    plane loads wrap around), which is exact for periodic ops at any distance.  A one-sided
+    or padded op differs only on the face layer itself (distance 0) and needs its input exact
+    within its reach; a periodic op over such an input needs that input's band plus its reach.
+    Voxels closer to a face than the band are evaluated by the gather path.
+    """
+    members = set(group.nodes)
+    sources = set(group.sources)
+    memo: Dict[str, int] = {}
+
+    def need(nid: str) -> int:
+        if nid in memo:
+            return memo[nid]
+        if nid in sources or nid not in members:
+            memo[nid] = 0
+            return 0
+        node = graph.node(nid)
+        steps = _op_reach_kind(node)
+        out = 0
+        for k, i in enumerate(node.input_ids):
+            if graph.node(i).op_name in REDUCTION_OPS and i not in sources:
+                continue
+            n = need(i)
+            my_steps = steps if not (node.op_name == "material_derivative" and k == 1) else []
+            for st in my_steps:
+                if st == "wrap":
+                    n = n + 1 if n > 0 else 0
+                else:                      # one-sided torch.gradient edges, padded laplacian
+                    n = max(1, n + 1)
+            out = max(out, n)
+        memo[nid] = out
+        return out
+
+    return max([need(n) for n in group.nodes] + [0])
+
+
 class KernelGen:
     """Builds the HIP source of one fused group for one schedule.
 
@@ -821,7 +920,7 @@ class KernelGen:
         self.dsc_index = {s: i for i, s in enumerate(group.dscalars)}
         reach = source_reach(graph, group)
         self.hz = max([r[0] for r in reach.values()] + [0])
-        self.redge = max([r[1] for r in reach.values()] + [0])
+        self.redge = edge_band(graph, group)
         self.use_lds = self.hz > 0 and len(group.sources) > 0
         if sched.threads % sched.tx != 0 or sched.ty % (sched.threads // sched.tx) != 0:
             raise ValueError(f"schedule {sched} does not map threads onto whole tile rows")
@@ -882,11 +981,38 @@ class KernelGen:
                 L_ = self.src_layout[nr.si]
                 L_.update(mode="ring", D=2 * self.hz + 2, lds=True)
 
+    def xface_sides(self) -> List[str]:
+        """x faces whose band the main kernel evaluates with compile-time face variants.
+
+        A face voxel sits in its own cache line per row and plane, so leaving the x faces to
+        the gather path re-reads whole lines long after the LDS kernel fetched them."""
+        if not self.use_lds or self.edge_mode != "band" or self.redge == 0:
+            return []
+        n2, tx, r = self.grid[2], self.sched.tx, self.redge
+        if n2 < tx or n2 < 2 * r + 1 or r > tx:
+            return []
+        # one extra wave per block evaluates them: RP rows x r columns x 2 faces must fit
+        if self.rp * r * 2 > 64 or self.sched.threads + 64 > 1024:
+            return []
+        sides = ["lo"]
+        if n2 % tx == 0:
+            sides.append("hi")
+        return sides
+
+    @property
+    def fw(self) -> int:
+        """Threads of the face wave (0 when the x faces are left to the tail blocks)."""
+        return 64 if self.xface_sides() else 0
+
+    def block_threads(self) -> int:
+        return self.sched.threads + self.fw
+
     def _variants(self):
         v = [("gather",)]
         if self.use_lds:
             if self.edge_mode == "band":
                 v.append(("lds", False, False))
+                v += [("xface", side, d) for side in self.xface_sides() for d in range(self.redge)]
             else:
                 v += [("lds", ex, ey) for ex in (False, True) for ey in (False, True)]
         return v
@@ -898,52 +1024,50 @@ class KernelGen:
             return []
         n0, n1, n2 = self.grid
         zlo, zhi, ylo, yhi, xlo, xhi = self.interior_bounds()
+        sides = self.xface_sides()
         boxes = [(0, zlo, 0, n1, 0, n2), (zhi, n0, 0, n1, 0, n2),
-                 (zlo, zhi, 0, ylo, 0, n2), (zlo, zhi, yhi, n1, 0, n2),
-                 (zlo, zhi, ylo, yhi, 0, xlo), (zlo, zhi, ylo, yhi, xhi, n2)]
+                 (zlo, zhi, 0, ylo, 0, n2), (zlo, zhi, yhi, n1, 0, n2)]
+        if "lo" not in sides:
+            boxes.append((zlo, zhi, ylo, yhi, 0, xlo))
+        if "hi" not in sides:
+            boxes.append((zlo, zhi, ylo, yhi, xhi, n2))
         return [b for b in boxes if b[1] > b[0] and b[3] > b[2] and b[5] > b[4]]
 
     def boundary_blocks(self) -> int:
         tot = sum((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]) for b in self.boundary_boxes())
-        return (tot + 255) // 256
+        return (tot + self.block_threads() - 1) // self.block_threads()
 
-    def _boundary_kernel(self, ndet: int) -> str:
+    def _boundary_part(self, A, ndet: int) -> None:
+        """Tail blocks (blockIdx.x >= p.nblocks): gather evaluation of the boundary boxes."""
         g = self.group
         boxes = self.boundary_boxes()
-        out: List[str] = []
-        A = out.append
-        A('extern "C" __global__ void __launch_bounds__(256) fused_boundary(const Params p, int rec0) {')
-        A("    __shared__ float red_scratch[4 * 256 / 64 + 4];")
-        A("    Acc A;")
-        for ri in range(len(g.reductions)):
-            A(f"    A.r{ri}.init();")
-        for i in range(ndet):
-            A(f"    A.dflag{i} = 0.0f;")
-        for i in range(len(g.dscalars)):
-            A(f"    A.sc{i} = (float)p.scal{i}[0];")
-        A("    const i64 t = (i64)blockIdx.x * 256 + threadIdx.x;")
-        A("    int z = -1, y = 0, x = 0;")
+        A("    if (b >= nb) {")
+        A("        const i64 t = (i64)(b - nb) * NTB + tid;")
+        A("        int z = -1, y = 0, x = 0;")
         off = 0
         for (z0, z1, y0, y1, x0, x1) in boxes:
             ny, nx = y1 - y0, x1 - x0
             size = (z1 - z0) * ny * nx
-            A(f"    if (z < 0 && t >= {off} && t < {off + size}) {{ const i64 q = t - {off}; "
+            A(f"        if (z < 0 && t >= {off} && t < {off + size}) {{ const i64 q = t - {off}; "
               f"x = {x0} + (int)(q % {nx}); y = {y0} + (int)((q / {nx}) % {ny}); z = {z0} + (int)(q / {nx * ny}); }}")
             off += size
         qa = "".join(", nullptr, nullptr, nullptr" for _ in self.queue_sources())
-        A("    if (z >= 0) {")
-        A("        const IDX vox = ((IDX)z * N1 + y) * N2 + x;")
-        A(f"        ev_gather(p, A, 0, 0, z, y, x, 0, 0, 0, 0, vox{qa});")
+        A("        if (z >= 0) {")
+        A("            const IDX vox = ((IDX)z * N1 + y) * N2 + x;")
+        A(f"            ev_gather(p, A, 0, 0, z, y, x, 0, 0, 0, 0, vox{qa});")
+        A("        }")
+        self._write_records(A, ndet, "        ")
+        A("        return;")
         A("    }")
-        for ri, nid in enumerate(g.reductions):
+
+    def _write_records(self, A, ndet: int, ind: str) -> None:
+        for ri, nid in enumerate(self.group.reductions):
             op = self.graph.node(nid).op_name
             fn = {"variance": "block_var", "min_max": "block_mm"}.get(op, "block_sum")
-            A(f"    {fn}(A.r{ri}, red_scratch, p.red{ri} + (i64)(rec0 + blockIdx.x) * 4);")
+            A(f"{ind}{fn}(A.r{ri}, red_scratch, p.red{ri} + (i64)blockIdx.x * 4);")
         for i in range(ndet):
-            A(f"    {{ RMinMax fm; fm.lo = INFINITY; fm.hi = A.dflag{i} > 0.0f ? 1.0f : -INFINITY;"
-              f" block_mm(fm, red_scratch, p.dflag{i} + (i64)(rec0 + blockIdx.x) * 4); }}")
-        A("}")
-        return "\n".join(out)
+            A(f"{ind}{{ RMinMax fm; fm.lo = INFINITY; fm.hi = A.dflag{i} > 0.0f ? 1.0f : -INFINITY;"
+              f" block_mm(fm, red_scratch, p.dflag{i} + (i64)blockIdx.x * 4); }}")
 
     def _walk(self, e: "_Emit") -> None:
         pos = e.origin()
@@ -1000,7 +1124,7 @@ class KernelGen:
             vals = e.val(nid, pos)
             body.append(f"if (fabsf({vals[0]}) < 1e-12f || fabsf({vals[4]}) < 1e-12f || "
                         f"fabsf({vals[8]}) < 1e-12f) A.dflag{ki} = 1.0f;")
-        name = "ev_gather" if mode[0] == "gather" else f"ev_lds{int(mode[1])}{int(mode[2])}"
+        name = self._variant_name(mode)
         code = "\n        ".join(e.lines + body)
         qargs = "".join(f", const float* qm{i}, const float* q0{i}, const float* qp{i}"
                         for i in self.queue_sources())
@@ -1009,6 +1133,25 @@ class KernelGen:
               f"        int y, int x, int ly, int lx, int y0, int x0, IDX vox{qargs}) {{\n"
               f"    {sc}\n        {code}\n}}\n")
         return name, fn
+
+    @staticmethod
+    def _variant_name(mode) -> str:
+        if mode[0] == "gather":
+            return "ev_gather"
+        if mode[0] == "xface":
+            return f"ev_x{mode[1]}{mode[2]}"
+        return f"ev_lds{int(mode[1])}{int(mode[2])}"
+
+    def main_boxes(self) -> List[Tuple[int, int, int, int, int, int]]:
+        """Boxes of voxels the main kernel evaluates (within full tiles)."""
+        if not self.use_lds:
+            return [(0, self.grid[0], 0, self.grid[1], 0, self.grid[2])]
+        zlo, zhi, ylo, yhi, xlo, xhi = self.interior_bounds()
+        out = [(zlo, zhi, ylo, yhi, xlo, xhi)]
+        n2 = self.grid[2]
+        for side in self.xface_sides():
+            out.append((zlo, zhi, ylo, yhi, 0, xlo) if side == "lo" else (zlo, zhi, ylo, yhi, n2 - self.redge, n2))
+        return out
 
     def _det_inputs(self) -> List[str]:
         out = []
@@ -1031,7 +1174,9 @@ class KernelGen:
         lines.append(f"#define N0 {n0}\n#define N1 {n1}\n#define N2 {n2}\n"
                      f"#define TX {s.tx}\n#define TY {s.ty}\n#define ZC {s.zc}\n#define NT {s.threads}\n"
                      f"#define HZ {self.hz}\n#define REDGE {self.redge}\n#define RP {self.rp}\n"
-                     f"#define VPT {self.v}\ntypedef {idx_t} IDX;\n")
+                     f"#define VPT {self.v}\ntypedef {idx_t} IDX;\n"
+                     f"#define FW {self.fw}\n#define NTB (NT + FW)\n"
+                     f"#define XLO_IN {int('lo' in self.xface_sides())}\n#define XHI_IN {int('hi' in self.xface_sides())}\n")
         if self.use_lds:
             _, _, ylo, yhi, xlo, xhi = self.interior_bounds()
             lines.append(f"#define YLO {ylo}\n#define YHI {yhi}\n#define XLO {xlo}\n#define XHI {xhi}\n")
@@ -1061,8 +1206,6 @@ class KernelGen:
             fns.append((m, name))
             lines.append(fn)
         lines.append(self._kernel(fns, ndet))
-        if self.use_lds:
-            lines.append(self._boundary_kernel(ndet))
         return "\n".join(lines)
 
     # -- kernel skeleton ---------------------------------------------------------------------
@@ -1070,17 +1213,11 @@ class KernelGen:
         s, g = self.sched, self.group
         out: List[str] = []
         A = out.append
-        A('extern "C" __global__ void __launch_bounds__(NT) fused_kernel(const Params p) {')
-        A("    __shared__ float red_scratch[4 * NT / 64 + 4];")
+        A('extern "C" __global__ void __launch_bounds__(NTB) fused_kernel(const Params p) {')
+        A("    __shared__ float red_scratch[4 * NTB / 64 + 4];")
         A("    const int tid = threadIdx.x;")
         # XCD-aware block remap: XCD k (hardware block id % 8) walks a contiguous tile range
         A("    const int b = blockIdx.x, nb = p.nblocks, per = nb >> 3, rem = nb & 7;")
-        A("    const int xcd = b & 7, lb = xcd * per + (xcd < rem ? xcd : rem) + (b >> 3);")
-        A("    const int tx_i = lb % p.tiles_x, t_ = lb / p.tiles_x;")
-        A("    const int ty_i = t_ % p.tiles_y, zc_i = t_ / p.tiles_y;")
-        A("    const int x0 = tx_i * TX, y0 = ty_i * TY;")
-        A("    const int zb = zc_i * ZC, ze = (zb + ZC < N0) ? zb + ZC : N0;")
-        A("    const int lx = tid % TX, ly0 = tid / TX;")
         A("    Acc A;")
         for ri in range(len(g.reductions)):
             A(f"    A.r{ri}.init();")
@@ -1088,6 +1225,27 @@ class KernelGen:
             A(f"    A.dflag{i} = 0.0f;")
         for i in range(len(g.dscalars)):
             A(f"    A.sc{i} = (float)p.scal{i}[0];")
+        if self.use_lds:
+            self._boundary_part(A, ndet)
+        A("    const int xcd = b & 7, lb = xcd * per + (xcd < rem ? xcd : rem) + (b >> 3);")
+        A("    const int tx_i = lb % p.tiles_x, t_ = lb / p.tiles_x;")
+        A("    const int ty_i = t_ % p.tiles_y, zc_i = t_ / p.tiles_y;")
+        A("    const int x0 = tx_i * TX, y0 = ty_i * TY;")
+        A("    const int zb = zc_i * ZC, ze = (zb + ZC < N0) ? zb + ZC : N0;")
+        A("    int lx = tid % TX, ly0 = tid / TX;")
+        if self.fw:
+            # the face wave (threads NT..NT+63): lane -> (face, column d, row phase); it loads
+            # its own face cells like any thread so that every wave runs the same memory code
+            A("    const bool fwave = tid >= NT;")
+            A("    bool fact = false;")
+            A("    if (fwave) {")
+            A("        const int fl = tid - NT, fs = fl / (RP * REDGE), fd = (fl / RP) % REDGE;")
+            A("        const bool hlo = XLO_IN && x0 == 0, hhi = XHI_IN && x0 + TX == N2;")
+            A("        fact = fs < (int)hlo + (int)hhi;")
+            A("        const bool lo = fs == 0 && hlo;")
+            A("        ly0 = fl % RP;")
+            A("        lx = !fact ? 0 : (lo ? fd : TX - 1 - fd);")
+            A("    }")
         A("    const bool ex = (x0 < REDGE) || (x0 + TX - 1 + REDGE > N2 - 1);")
         A("    const bool ey = (y0 < REDGE) || (y0 + TY - 1 + REDGE > N1 - 1);")
         # partial tiles (last tile of an axis not divisible by the tile) use the gather
@@ -1118,13 +1276,7 @@ class KernelGen:
         self._emit_step(A, names, "B", 1, zname="z_")
         A("        }")
         A("    }")
-        for ri, nid in enumerate(g.reductions):
-            op = self.graph.node(nid).op_name
-            fn = {"variance": "block_var", "min_max": "block_mm"}.get(op, "block_sum")
-            A(f"    {fn}(A.r{ri}, red_scratch, p.red{ri} + (i64)blockIdx.x * 4);")
-        for i in range(ndet):
-            A(f"    {{ RMinMax fm; fm.lo = INFINITY; fm.hi = A.dflag{i} > 0.0f ? 1.0f : -INFINITY;"
-              f" block_mm(fm, red_scratch, p.dflag{i} + (i64)blockIdx.x * 4); }}")
+        self._write_records(A, ndet, "    ")
         A("}")
         return "\n".join(out)
 
@@ -1153,7 +1305,7 @@ class KernelGen:
                 A(f"    int hg{i}[{KH}], hl{i}[{KH}];")
                 A("    #pragma unroll")
                 A(f"    for (int j = 0; j < {KH}; ++j) {{")
-                A(f"        const int e = tid + j * NT;")
+                A(f"        const int e = tid < NT ? tid + j * NT : {HT};")
                 A(f"        int r, col, ch;")
                 top = H * W * C
                 mid = s.ty * 2 * H * C
@@ -1229,6 +1381,8 @@ class KernelGen:
         z = zname
         A(f"{ind}{{ const int s = {z} - zb + HZ;")
         A(f"{ind}  const bool ez = ({z} < REDGE) || ({z} + REDGE > N0 - 1);")
+        if self.fw:
+            A(f"{ind}  if (!fwave) {{")
         A(f"{ind}  #pragma unroll")
         A(f"{ind}  for (int k = 0; k < VPT; ++k) {{")
         A(f"{ind}    const int ly = ly0 + k * RP, y = y0 + ly, x = x0 + lx;")
@@ -1237,7 +1391,8 @@ class KernelGen:
         qa = "".join(f", qm{i}[k], q0{i}[k], {buf}o{i}[k]" for i in self.queue_sources())
         call = f"(p, A, s, {sq}, {z}, y, x, ly, lx, y0, x0, vox{qa});"
         if self.edge_mode == "band" and ("lds", False, False) in names:
-            # z-edge planes and the x/y face band belong to the boundary kernel
+            # z-edge planes, the y face band and corners belong to the tail blocks; the
+            # x-face band to the face wave
             A(f"{ind}      if (!ez && y >= YLO && y < YHI && x >= XLO && x < XHI) ev_lds00{call}")
         elif ("lds", False, False) in names:
             # z-edge planes belong to the boundary kernel
@@ -1250,6 +1405,22 @@ class KernelGen:
             A(f"{ind}      ev_gather{call}")
         A(f"{ind}    }}")
         A(f"{ind}  }}")
+        if self.fw:
+            A(f"{ind}  }} else if (fact) {{")
+            A(f"{ind}  #pragma unroll")
+            A(f"{ind}  for (int k = 0; k < VPT; ++k) {{")
+            A(f"{ind}    const int ly = ly0 + k * RP, y = y0 + ly, x = x0 + lx;")
+            A(f"{ind}    if (!ez && y >= YLO && y < YHI) {{")
+            A(f"{ind}      const IDX vox = ((IDX){z} * N1 + y) * N2 + x;")
+            first = True
+            for side in self.xface_sides():
+                for d in range(self.redge):
+                    xc = d if side == "lo" else self.grid[2] - 1 - d
+                    A(f"{ind}      {'if' if first else 'else if'} (x == {xc}) ev_x{side}{d}{call}")
+                    first = False
+            A(f"{ind}    }}")
+            A(f"{ind}  }}")
+            A(f"{ind}  }}")
         if self.use_lds:
             for i, L_ in enumerate(self.src_layout):
                 slot = str(sq ^ 1) if L_["mode"] == "queue" else f"(unsigned)(s + HZ + 1) % {L_['D']}u"
@@ -1320,17 +1491,10 @@ class CompiledGroup:
     gen: KernelGen
     source: str
     module: Optional[L.Module] = None
-    bmodule: Optional[L.Module] = None
 
     @property
     def group(self) -> Group:
         return self.gen.group
-
-    def boundary_module(self) -> L.Module:
-        if self.bmodule is None:
-            code, _ = compile_source(self.source)
-            self.bmodule = L.Module(code, "fused_boundary")
-        return self.bmodule
 
     def nblocks(self) -> Tuple[int, int, int, int]:
         s = self.gen.sched
@@ -1510,11 +1674,8 @@ class Launch:
     def launch(self, stream: Optional[int] = None, det_reg: float = 0.0) -> None:
         st = stream if stream is not None else L.stream_ptr()
         a = self.args(det_reg)
-        self.cg.module.launch((self.nblocks, 1, 1), (self.cg.gen.sched.threads, 1, 1), a, 0, st)
-        nbb = self.cg.gen.boundary_blocks()
-        if nbb:
-            self.cg.boundary_module().launch((nbb, 1, 1), (256, 1, 1),
-                                             a + struct.pack("<i", self.nblocks), 0, st)
+        grid = self.nblocks + self.cg.gen.boundary_blocks()
+        self.cg.module.launch((grid, 1, 1), (self.cg.gen.block_threads(), 1, 1), a, 0, st)
 
     def time_ms(self, reps: int = 20, warmup: int = 3, rounds: int = 3) -> float:
         """Min over ``rounds`` of the mean kernel time, HIP events on the launch stream."""

@@ -67,6 +67,8 @@ def test_generated_kernels_compile(expr, bc):
 
 @pytest.mark.parametrize("edge_mode", ["band", "variants"])
 @pytest.mark.parametrize("shape,sched", [((20, 24, 30), F.Schedule(32, 8, 16, 256)),
+                                         ((20, 24, 64), F.Schedule(32, 8, 16, 256)),
+                                         ((12, 40, 128), F.Schedule(64, 8, 64, 256)),
                                          ((9, 8, 130), F.Schedule(64, 8, 64, 256)),
                                          ((21, 37, 70), F.Schedule(16, 4, 64, 64)),
                                          ((40, 45, 150), F.Schedule(128, 4, 32, 256))])
@@ -81,15 +83,16 @@ def test_main_and_boundary_kernels_cover_every_voxel_once(shape, sched, edge_mod
     n0, n1, n2 = shape
     hits = torch.zeros(shape, dtype=torch.int32)
     z0, z1, y0, y1, x0, x1 = gen.interior_bounds()
-    # the main kernel: full tiles only, interior box within them
-    for ty in range(0, n1, sched.ty):
-        for tx in range(0, n2, sched.tx):
-            if ty + sched.ty > n1 or tx + sched.tx > n2:
-                continue
-            ya, yb = max(ty, y0), min(ty + sched.ty, y1)
-            xa, xb = max(tx, x0), min(tx + sched.tx, x1)
-            if ya < yb and xa < xb and z0 < z1:
-                hits[z0:z1, ya:yb, xa:xb] += 1
+    # the main kernel: full tiles only, its boxes within them
+    for (c0, c1, b0, b1, a0, a1) in gen.main_boxes():
+        for ty in range(0, n1, sched.ty):
+            for tx in range(0, n2, sched.tx):
+                if ty + sched.ty > n1 or tx + sched.tx > n2:
+                    continue
+                ya, yb = max(ty, b0), min(ty + sched.ty, b1)
+                xa, xb = max(tx, a0), min(tx + sched.tx, a1)
+                if ya < yb and xa < xb and c0 < c1:
+                    hits[c0:c1, ya:yb, xa:xb] += 1
     for (a0, a1, b0, b1, c0, c1) in gen.boundary_boxes():
         hits[a0:a1, b0:b1, c0:c1] += 1
     assert int(hits.min()) == 1 and int(hits.max()) == 1
@@ -97,5 +100,5 @@ def test_main_and_boundary_kernels_cover_every_voxel_once(shape, sched, edge_mod
     r = gen.redge
     assert z0 >= r and z1 <= n0 - r or z0 >= z1
     assert y0 >= min(r, n1) * (edge_mode == "band") and x0 >= min(r, n2) * (edge_mode == "band")
-    assert gen.boundary_blocks() * 256 >= sum((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4])
+    assert gen.boundary_blocks() * gen.block_threads() >= sum((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4])
                                               for b in gen.boundary_boxes())
