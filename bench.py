@@ -79,6 +79,129 @@ def cpu_baseline(n: int, planes: int):
     return vox / ts[1], threads, f"{planes}x{n}x{n} slab, FP64 torch CPU oracle, median of 3"
 
 
+def _line(metric, value, unit, steps, warmup, ms, config, **extra):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+         "warmup": warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Fourier fields)",
+         "config": config}
+    d.update(extra)
+    print(json.dumps(d), flush=True)
+
+
+def bench_c3(args) -> int:
+    """96 fields, axes U{32..384} (seed 6), 6 random graphs of 3-12 ops (seed 6); field i uses
+    graph i mod 6 and seed 1000 + i; every level one persistent launch, replayed as a hipGraph."""
+    import random
+    import numpy as np
+    import torch
+    import stenpy_amd as sp
+    from stenpy_amd import batch as B, gen, randgraph, tune
+    dev = torch.device("cuda:0")
+    rng = random.Random(6)
+    exprs = [randgraph.random_expression(rng, rng.randint(3, 12), vector_leaf=False, mid_mean=0.15)[0]
+             for _ in range(6)]
+    shapes = [tuple(rng.randint(32, 384) for _ in range(3)) for _ in range(96)]
+    items = []
+    for i, sh in enumerate(shapes):
+        f = gen.generate(sh, 1000 + i, dev)
+        g, sink, *_ = sp.compile_expression(exprs[i % 6], boundary="periodic", field_map={"f": f})
+        items.append((g, [sink]))
+    t0 = time.perf_counter()
+    br = B.BatchRunner(items, device=dev)
+    br.compile(workers=8)
+    t_plan = time.perf_counter() - t0
+    graph = br.capture()
+    st = br.stats()
+
+    def step():
+        graph.replay()
+    t = time_steps(step, args.steps, args.warmup)
+    # the same work one graph at a time through the fused executor (no batching, no capture)
+    rt = sp.Runtime(device=dev, fusion_enabled=True)
+    seq = lambda: [rt.fused.run(g, rt, wanted=w) for g, w in items]
+    t_seq = time_steps(seq, max(2, args.steps // 5), 1)
+    vox = st["voxels"]
+    _line("C3 batched persistent launch: Gvoxel/s over 96 uneven fields x 6 random graphs",
+          round(vox / t / 1e9, 2), "Gvoxel/s", args.steps, args.warmup, t * 1e3,
+          {"workload": "96 fields, axes U{32..384} seed 6; 6 random graphs (3-12 ops) seed 6; periodic",
+           "exprs": exprs, "levels": st["levels"], "units": st["units"], "structures": st["structures"],
+           "voxels": vox, "plan_compile_s": round(t_plan, 2),
+           "sequential_fused_ms": round(t_seq * 1e3, 3), "speedup_vs_sequential": round(t_seq / t, 2)},
+          roofline={"bound": "hbm", "achieved": round(st["algorithmic_bytes"] / t / 1e9, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(st["algorithmic_bytes"] / t / 1e9 / HBM_PEAK_GBS, 4),
+                    "traffic": None, "algorithmic_bytes_per_launch": st["algorithmic_bytes"]})
+    return 0
+
+
+def bench_c5(args) -> int:
+    """768^3: lower half seeded Fourier (seed 8), upper half i^2 + j^2 + k^2; hessian ->
+    eigenvalues, determinant, trace, deviatoric, mean curvature, fused."""
+    import torch
+    import stenpy_amd as sp
+    from stenpy_amd import gen, tune
+    dev = torch.device("cuda:0")
+    n = 768 if args.n == 512 else args.n
+    f = gen.generate((n, n, n), 8, dev)
+    i = torch.arange(n, device=dev, dtype=torch.float32)
+    q = (i.view(-1, 1, 1) ** 2 + i.view(1, -1, 1) ** 2 + i.view(1, 1, -1) ** 2)
+    f[n // 2:] = q[n // 2:]
+    del q
+    exprs = ["eigenvalues(hessian(f))", "determinant(hessian(f))", "trace(hessian(f))",
+             "deviatoric(hessian(f))", "mean_curvature(f)"]
+    from stenpy_amd.compiler import compile_expression
+    # one graph, five sinks sharing the hessian: f is read once per evaluation
+    g, eig, *_ = compile_expression(exprs[0], boundary="periodic", field_map={"f": f})
+    hes = g.node(eig).input_ids[0]
+    src = g.node(hes).input_ids[0]
+    sinks = [eig] + [g.add(op, (hes,), {}) for op in ("determinant", "trace", "deviatoric")]
+    sinks.append(g.add("mean_curvature", (src,), dict(g.node(hes).params, boundary="periodic")))
+    graphs = [(g, sinks)]
+    rt = sp.Runtime(device=dev, fusion_enabled=True, schedule_fn=tune.schedule_fn(dev))
+
+    def step():
+        for gg, w in graphs:
+            rt.fused.run(gg, rt, wanted=w)
+    t = time_steps(step, max(3, args.steps // 4), 1)
+    nvox = n ** 3
+    alg = nvox * 4 * (1 + 3 + 1 + 1 + 9 + 1)     # read f once, write 15 output channels (64 B)
+    _line("C5 tensor ops: Gvoxel/s of hessian -> eig/det/trace/dev + mean curvature on 768^3",
+          round(nvox / t / 1e9, 3), "Gvoxel/s", max(3, args.steps // 4), 1, t * 1e3,
+          {"workload": f"{n}^3 FP32, lower half Fourier seed 8, upper half i^2+j^2+k^2; periodic; "
+                       "one graph with five sinks sharing the hessian", "exprs": exprs,
+           "groups": len(rt.fused._plans and next(iter(rt.fused._plans.values()))[2])},
+          roofline={"bound": "hbm", "achieved": round(alg / t / 1e9, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(alg / t / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                    "algorithmic_bytes_per_launch": alg})
+    return 0
+
+
+def bench_c4(args) -> int:
+    """4096 x 4096 x 4608 field generated on the host slab by slab (256 planes, 1-plane halo),
+    streamed through pinned buffers; a bounded sample of ``--slabs`` slabs is timed."""
+    import torch
+    import stenpy_amd as sp
+    from stenpy_amd.stream import GeneratorField, stream_graph
+    from stenpy_amd.compiler import compile_expression
+    dev = torch.device("cuda:0")
+    shape = (4608, 4096, 4096)
+    lazy = GeneratorField(shape, 7)
+    probe = torch.zeros((1,) + shape[1:])
+    g, sink, _, fm, _ = compile_expression("f**2 + tanh(f)*exp(f) + laplacian(f)", boundary="periodic",
+                                           field_map={"f": probe})
+    rt = sp.Runtime(device=dev, fusion_enabled=True)
+    st = stream_graph(g, sink, fm["f"], lazy, None, rt, boundary="periodic", slab_planes=256,
+                      max_slabs=args.slabs)
+    vox = st["voxels"]
+    _line("C4 out-of-core stream: Gvoxel/s (host generation + H2D + fused compute), bounded sample",
+          round(st["gvoxel_s"], 3), "Gvoxel/s", 1, 0, st["wall_s"] * 1e3,
+          {"workload": "4608x4096x4096 FP32 seed 7, 256-plane slabs, periodic halo 1; "
+                       f"sample = first {st['n_chunks']} slabs", "slabs": st["n_chunks"],
+           "host_gen_s": round(st["host_gen_s"], 2), "l2_of_sample": st["l2"],
+           "h2d_gbs": round(vox * 4 / st["wall_s"] / 1e9, 2)})
+    return 0
+
+
 def copy_bytes(n: int) -> int:
     return 4 * n ** 3
 
@@ -144,6 +267,11 @@ def main() -> int:
                     help="render a rocprofv3 --stats CSV output as markdown (no GPU)")
     ap.add_argument("--cmd", type=str, default="", help="command line quoted in --stats-md output")
     ap.add_argument("--out", type=str, default=None)
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
+                    help="c2: the headline 512^3 graph (default); c3: 96 uneven fields x 6 random "
+                         "graphs in batched persistent launches; c4: out-of-core stream (bounded "
+                         "sample of slabs); c5: 768^3 hessian tensor ops")
+    ap.add_argument("--slabs", type=int, default=2, help="c4: slabs of 256 planes to stream")
     args = ap.parse_args()
     if args.pmc_traffic or args.stats_md:
         return offline_reports(args)
@@ -152,6 +280,8 @@ def main() -> int:
         return 2
     if args.profile_pass:
         return profile_pass(args)
+    if args.workload != "c2":
+        return {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)
 
     import torch
     import stenpy_amd as sp

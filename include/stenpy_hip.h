@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define SP_ABI_VERSION 3
+#define SP_ABI_VERSION 4
 
 /* status codes */
 enum { SP_OK = 0, SP_EINVAL = 1, SP_EHIP = 2, SP_EUNSUPPORTED = 3, SP_ECOMPILE = 4 };
@@ -134,6 +134,17 @@ int sp_reduce_combine(int kind, const double* records, int nrecords, int64_t cou
                       double* result, void* stream);
 int sp_reduce(int kind, const sp_tensor* x, double* result, void* workspace,
               const double* wparams, void* stream);
+/* Many final combines in one launch (one block each): ``descs`` is a device array of n
+ * descriptors, each as for sp_reduce_combine.  Used after a batched launch of many grids.
+ * No counterpart in the reference (it reduces one tensor per call, engine 2340-2382). */
+typedef struct sp_combine_desc {
+    const double* records;
+    double* result;
+    int64_t count;
+    int32_t kind;
+    int32_t nrecords;
+} sp_combine_desc;
+int sp_reduce_combine_batch(const sp_combine_desc* descs, int n, void* stream);
 
 /* ---- per-voxel symmetric 3x3 tensor ops (stenpy_engine.py:2674-2730, 2852-2867) -------
  *   SP_T_TRACE -> (...), SP_T_DET -> (...) [``flag`` = device double from SP_R_ANYSMALLDIAG:
@@ -172,6 +183,9 @@ int  sp_module_launch(sp_module* mod, const unsigned grid[3], const unsigned blo
 /* registers, LDS bytes and spill bytes of the loaded kernel */
 int  sp_module_attributes(sp_module* mod, int* vgprs, int* lds_bytes, int* spill_bytes,
                           int* max_threads);
+/* Blocks of ``block_threads`` threads with ``dyn_shared`` bytes of dynamic LDS that fit on
+ * one CU at once (sizes the grid of a persistent batch launch). */
+int  sp_module_occupancy(sp_module* mod, int block_threads, size_t dyn_shared, int* blocks_per_cu);
 void sp_module_unload(sp_module* mod);
 
 /* ---- schedule model (MLP trained on autotuner timings) --------------------------------

@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "libstenpy_hip.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # enums mirrored from include/stenpy_hip.h
 SP_F32, SP_BF16, SP_F64 = 0, 1, 2
@@ -86,6 +86,7 @@ def _declare(lib: C.CDLL) -> None:
         "sp_reduce_workspace_bytes": ([C.c_int64], Z),
         "sp_reduce": ([I, P, V, V, C.POINTER(D), V], I),
         "sp_reduce_combine": ([I, V, I, C.c_int64, V, V], I),
+        "sp_reduce_combine_batch": ([V, I, V], I),
         "sp_tensor3": ([I, P, P, V, V], I),
         "sp_gen_fourier": ([P, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int32),
                             C.POINTER(D), C.POINTER(D), C.c_uint64, V], I),
@@ -96,6 +97,7 @@ def _declare(lib: C.CDLL) -> None:
         "sp_module_launch": ([V, C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.c_uint, V, V, Z], I),
         "sp_module_attributes": ([V, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)], I),
         "sp_module_unload": ([V], None),
+        "sp_module_occupancy": ([V, I, C.c_size_t, C.POINTER(I)], I),
         "sp_host_gen_fourier": ([V, C.POINTER(C.c_int64), C.c_int64, C.c_int64, C.POINTER(C.c_int32),
                                  C.POINTER(D), C.POINTER(D), C.c_uint64, I], I),
         "sp_gemm_bf16": ([I, I, I, I, I, V, I, V, I, V, I, C.c_float, I, V], I),
@@ -315,6 +317,13 @@ class Module:
         b = (C.c_uint * 3)(*block)
         s = stream if stream is not None else stream_ptr()
         check(lib().sp_module_launch(self.handle, g, b, shared, s, args, len(args)), "launch")
+
+    def occupancy(self, block_threads: int, dyn_shared: int = 0) -> int:
+        """Blocks per CU that fit at once (needs a device)."""
+        n = C.c_int()
+        check(lib().sp_module_occupancy(self.handle, int(block_threads), int(dyn_shared), C.byref(n)),
+              "occupancy")
+        return n.value
 
     def attributes(self):
         v, l, s, m = C.c_int(), C.c_int(), C.c_int(), C.c_int()

@@ -225,6 +225,15 @@ extern "C" int sp_module_attributes(sp_module* mod, int* vgprs, int* lds_bytes,
     return SP_OK;
 }
 
+extern "C" int sp_module_occupancy(sp_module* mod, int block_threads, size_t dyn_shared,
+                                   int* blocks_per_cu) {
+    SP_REQUIRE(mod && mod->fn && blocks_per_cu && block_threads > 0, "module_occupancy: bad arguments");
+    int n = 0;
+    SP_HIP_CHECK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&n, mod->fn, block_threads, dyn_shared));
+    *blocks_per_cu = n;
+    return SP_OK;
+}
+
 extern "C" void sp_module_unload(sp_module* mod) {
     if (!mod) return;
     if (mod->module) hipModuleUnload(mod->module);

@@ -190,9 +190,8 @@ __device__ __forceinline__ void merge64(Acc64& a, const Acc64& b, bool minmax) {
     a.c2 += b.c2;
 }
 
-__global__ void __launch_bounds__(kThreads) k_reduce_stage2(int kind, const double* __restrict__ ws,
-                                                            int nblocks, int64_t n,
-                                                            double* __restrict__ result) {
+__device__ __forceinline__ void combine_records(int kind, const double* __restrict__ ws, int nblocks,
+                                                int64_t n, double* __restrict__ result) {
     const bool mm = (kind == SP_R_MINMAX || kind == SP_R_ANYSMALLDIAG);
     Acc64 a{0, 0, 0, 0, INFINITY, -INFINITY};
     for (int b = threadIdx.x; b < nblocks; b += kThreads) {
@@ -227,6 +226,18 @@ __global__ void __launch_bounds__(kThreads) k_reduce_stage2(int kind, const doub
             case SP_R_ANYSMALLDIAG: result[0] = r.hi > 0.0 ? 1.0 : 0.0; break;
         }
     }
+}
+
+__global__ void __launch_bounds__(kThreads) k_reduce_stage2(int kind, const double* __restrict__ ws,
+                                                            int nblocks, int64_t n,
+                                                            double* __restrict__ result) {
+    combine_records(kind, ws, nblocks, n, result);
+}
+
+// one block per descriptor: the final combines of many reductions in one launch
+__global__ void __launch_bounds__(kThreads) k_reduce_combine_batch(const sp_combine_desc* __restrict__ d) {
+    const sp_combine_desc e = d[blockIdx.x];
+    combine_records(e.kind, e.records, e.nrecords, e.count, e.result);
 }
 
 }  // namespace
@@ -275,4 +286,11 @@ extern "C" int sp_reduce_combine(int kind, const double* records, int nrecords, 
     hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(kThreads), 0, as_stream(stream), kind,
                        records, nrecords, count, result);
     return launch_status("sp_reduce_combine");
+}
+
+extern "C" int sp_reduce_combine_batch(const sp_combine_desc* descs, int n, void* stream) {
+    SP_REQUIRE(descs && n > 0, "reduce_combine_batch: bad arguments");
+    hipLaunchKernelGGL(k_reduce_combine_batch, dim3((unsigned)n), dim3(kThreads), 0,
+                       as_stream(stream), descs);
+    return launch_status("sp_reduce_combine_batch");
 }

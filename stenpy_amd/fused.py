@@ -82,6 +82,22 @@ class Schedule:
 
 
 DEFAULT_SCHEDULE = Schedule()
+
+
+def heuristic_schedule(grid: Sequence[int]) -> Schedule:
+    """Shape-only schedule used when no tuned or learned schedule is available.
+
+    Wide tiles cut the x-halo share, VPT = 2 rows per thread, and z-chunks long enough that
+    the halo planes of a chunk are a small share while leaving >= ~2 blocks per CU."""
+    n0, n1, n2 = (int(v) for v in grid)
+    tx = 128 if n2 >= 384 else 64 if n2 >= 96 else 32 if n2 >= 32 else 16
+    ty = 8 if n1 >= 16 else 4
+    nt = min(512, max(64, tx * ty // 2))
+    tiles = -(-n2 // tx) * -(-n1 // ty)
+    zc = 256
+    while zc > 32 and (zc >= n0 or tiles * -(-n0 // zc) < 512):
+        zc //= 2
+    return Schedule(tx, ty, zc, nt)
 # LDS per block kept within 64 KiB so that at least two blocks share a CU
 LDS_LIMIT = 64 * 1024
 FALLBACK_SCHEDULES = [Schedule(64, 4, 64, 256), Schedule(32, 8, 64, 256), Schedule(32, 4, 64, 128),
@@ -91,6 +107,9 @@ FALLBACK_SCHEDULES = [Schedule(64, 4, 64, 256), Schedule(32, 8, 64, 256), Schedu
 # face band goes to the gather kernel.  "variants": tiles touching an x/y face run
 # edge-specialised LDS variants.  "auto" picks "band" when the band is a small share.
 EDGE_MODE = os.environ.get("STENPY_EDGE_MODE", "auto")
+# Tail blocks (gather evaluation of the face band) at the start of the grid instead of the end
+# (measured slower on the 512^3 graph; kept as an option).
+TAIL_FIRST = os.environ.get("STENPY_TAIL_FIRST", "0") == "1"
 BAND_MAX_FRACTION = 0.08
 
 
@@ -912,7 +931,7 @@ class KernelGen:
 
     def __init__(self, graph: Graph, group: Group, info: Dict[str, Info], sched: Schedule,
                  out_dtypes: Dict[str, torch.dtype], src_meta: Dict[str, Tuple[int, ...]],
-                 edge_mode: Optional[str] = None):
+                 edge_mode: Optional[str] = None, batched: bool = False):
         self.graph, self.group, self.info, self.sched = graph, group, info, sched
         self.grid = group.grid
         self.out_dtypes = out_dtypes
@@ -928,6 +947,14 @@ class KernelGen:
         self.v = sched.ty // self.rp                 # voxels per thread per plane
         self.src_layout = [{"C": info[s].ch, "H": reach[s][2], "D": 2 * self.hz + 2,
                             "mode": "ring", "lds": True} for s in group.sources]
+        # batched: one code for many grids (shapes are run-time values, see section())
+        self.batched = batched
+        if batched:
+            if min(self.grid) < 2 * self.redge + 1 or min(self.grid) < 3:
+                raise ValueError("batched groups need every axis >= max(3, 2 * band + 1)")
+            if self._det_inputs():
+                raise ValueError("determinant groups are not batched (their relaunch is per grid)")
+            edge_mode = "band"
         self.edge_mode = self._pick_edge_mode(edge_mode or EDGE_MODE)
         self._choose_modes()
 
@@ -941,9 +968,9 @@ class KernelGen:
         frac = 1.0 - (n1 - 2 * r) * (n2 - 2 * r) / float(n1 * n2)
         return "band" if frac <= BAND_MAX_FRACTION else "variants"
 
-    def interior_bounds(self) -> Tuple[int, int, int, int, int, int]:
+    def interior_bounds(self, grid=None) -> Tuple[int, int, int, int, int, int]:
         """Box (z0, z1, y0, y1, x0, x1) of voxels the main kernel evaluates."""
-        n0, n1, n2 = self.grid
+        n0, n1, n2 = grid or self.grid
         r, s = self.redge, self.sched
         xp = (n2 // s.tx) * s.tx          # start of a partial x tile (== n2 if none)
         yp = (n1 // s.ty) * s.ty
@@ -986,7 +1013,7 @@ class KernelGen:
 
         A face voxel sits in its own cache line per row and plane, so leaving the x faces to
         the gather path re-reads whole lines long after the LDS kernel fetched them."""
-        if not self.use_lds or self.edge_mode != "band" or self.redge == 0:
+        if not self.use_lds or self.edge_mode != "band" or self.redge == 0 or self.batched:
             return []
         n2, tx, r = self.grid[2], self.sched.tx, self.redge
         if n2 < tx or n2 < 2 * r + 1 or r > tx:
@@ -1018,12 +1045,12 @@ class KernelGen:
         return v
 
     # -- boundary kernel -------------------------------------------------------------------------
-    def boundary_boxes(self) -> List[Tuple[int, int, int, int, int, int]]:
+    def boundary_boxes(self, grid=None) -> List[Tuple[int, int, int, int, int, int]]:
         """Disjoint boxes (z0, z1, y0, y1, x0, x1) the main kernel leaves out."""
         if not self.use_lds:
             return []
-        n0, n1, n2 = self.grid
-        zlo, zhi, ylo, yhi, xlo, xhi = self.interior_bounds()
+        n0, n1, n2 = grid or self.grid
+        zlo, zhi, ylo, yhi, xlo, xhi = self.interior_bounds(grid)
         sides = self.xface_sides()
         boxes = [(0, zlo, 0, n1, 0, n2), (zhi, n0, 0, n1, 0, n2),
                  (zlo, zhi, 0, ylo, 0, n2), (zlo, zhi, yhi, n1, 0, n2)]
@@ -1033,17 +1060,33 @@ class KernelGen:
             boxes.append((zlo, zhi, ylo, yhi, xhi, n2))
         return [b for b in boxes if b[1] > b[0] and b[3] > b[2] and b[5] > b[4]]
 
-    def boundary_blocks(self) -> int:
-        tot = sum((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]) for b in self.boundary_boxes())
-        return (tot + self.block_threads() - 1) // self.block_threads()
+    def boundary_blocks(self, grid=None) -> int:
+        tot = sum((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]) for b in self.boundary_boxes(grid))
+        n = (tot + self.block_threads() - 1) // self.block_threads()
+        if self.batched:
+            return n
+        # a multiple of 8 keeps the main blocks' hardware ids congruent to their XCD index
+        return (n + 7) // 8 * 8 if (TAIL_FIRST and n) else n
 
     def _boundary_part(self, A, ndet: int) -> None:
         """Tail blocks (blockIdx.x >= p.nblocks): gather evaluation of the boundary boxes."""
         g = self.group
         boxes = self.boundary_boxes()
-        A("    if (b >= nb) {")
-        A("        const i64 t = (i64)(b - nb) * NTB + tid;")
+        A("    if (TAIL_FIRST ? b < NBT : b >= nb) {")
+        A("        const i64 t = (i64)(TAIL_FIRST ? b : b - nb) * NTB + tid;")
         A("        int z = -1, y = 0, x = 0;")
+        if self.batched:
+            # boxes of this grid, from the parameters: bstart = prefix sums of box sizes
+            A("        #pragma unroll")
+            A("        for (int k = 0; k < 6; ++k) {")
+            A("            if (z < 0 && t >= p.bstart[k] && t < p.bstart[k + 1]) {")
+            A("                const int* bx = p.boxes + 6 * k;")
+            A("                const i64 q = t - p.bstart[k];")
+            A("                const int ny = bx[3] - bx[2], nx = bx[5] - bx[4];")
+            A("                x = bx[4] + (int)(q % nx); y = bx[2] + (int)((q / nx) % ny); z = bx[0] + (int)(q / ((i64)nx * ny));")
+            A("            }")
+            A("        }")
+            boxes = []
         off = 0
         for (z0, z1, y0, y1, x0, x1) in boxes:
             ny, nx = y1 - y0, x1 - x0
@@ -1064,10 +1107,10 @@ class KernelGen:
         for ri, nid in enumerate(self.group.reductions):
             op = self.graph.node(nid).op_name
             fn = {"variance": "block_var", "min_max": "block_mm"}.get(op, "block_sum")
-            A(f"{ind}{fn}(A.r{ri}, red_scratch, p.red{ri} + (i64)blockIdx.x * 4);")
+            A(f"{ind}{fn}(A.r{ri}, red_scratch, p.red{ri} + (i64)rb * 4);")
         for i in range(ndet):
             A(f"{ind}{{ RMinMax fm; fm.lo = INFINITY; fm.hi = A.dflag{i} > 0.0f ? 1.0f : -INFINITY;"
-              f" block_mm(fm, red_scratch, p.dflag{i} + (i64)blockIdx.x * 4); }}")
+              f" block_mm(fm, red_scratch, p.dflag{i} + (i64)rb * 4); }}")
 
     def _walk(self, e: "_Emit") -> None:
         pos = e.origin()
@@ -1085,12 +1128,7 @@ class KernelGen:
         return [i for i, L_ in enumerate(self.src_layout) if self.use_lds and L_["lds"]]
 
     def lds_bytes(self) -> int:
-        s = self.sched
-        tot = 0
-        for i in self.lds_sources():
-            L_ = self.src_layout[i]
-            tot += L_["D"] * (L_["C"] + 1) * (s.ty + 2 * L_["H"]) * (s.tx + 2 * L_["H"]) * 4
-        return tot + 4 * (s.threads // 64) * 4 + 16
+        return self.lds_array_bytes() + 4 * (self.block_threads() // 64) * 4 + 16
 
     # -- per-voxel code ------------------------------------------------------------------------
     def _emit_variant(self, mode) -> Tuple[str, str]:
@@ -1142,13 +1180,14 @@ class KernelGen:
             return f"ev_x{mode[1]}{mode[2]}"
         return f"ev_lds{int(mode[1])}{int(mode[2])}"
 
-    def main_boxes(self) -> List[Tuple[int, int, int, int, int, int]]:
+    def main_boxes(self, grid=None) -> List[Tuple[int, int, int, int, int, int]]:
         """Boxes of voxels the main kernel evaluates (within full tiles)."""
+        grid = grid or self.grid
         if not self.use_lds:
-            return [(0, self.grid[0], 0, self.grid[1], 0, self.grid[2])]
-        zlo, zhi, ylo, yhi, xlo, xhi = self.interior_bounds()
+            return [(0, grid[0], 0, grid[1], 0, grid[2])]
+        zlo, zhi, ylo, yhi, xlo, xhi = self.interior_bounds(grid)
         out = [(zlo, zhi, ylo, yhi, xlo, xhi)]
-        n2 = self.grid[2]
+        n2 = grid[2]
         for side in self.xface_sides():
             out.append((zlo, zhi, ylo, yhi, 0, xlo) if side == "lo" else (zlo, zhi, ylo, yhi, n2 - self.redge, n2))
         return out
@@ -1161,23 +1200,60 @@ class KernelGen:
                 out.append(node.input_ids[0])
         return out
 
+    # macros a generated section defines (undefined again at the end of a batched section)
+    _SECTION_MACROS = ("N0", "N1", "N2", "TX", "TY", "ZC", "NT", "HZ", "REDGE", "RP", "VPT", "FW", "NTB",
+                       "NBT", "TAIL_FIRST", "XLO_IN", "XHI_IN", "YLO", "YHI", "XLO", "XHI")
+
     def source(self) -> str:
+        if self.batched:
+            raise ValueError("a batched generator emits sections (section()), not a module")
+        return "\n".join(self._header_sources() + self._body())
+
+    @staticmethod
+    def _header_sources() -> List[str]:
+        return [(CSRC / hdr).read_text().replace("#pragma once", "") for hdr in ("sp_sym3.h", "fused_template.h")]
+
+    def section(self, ns: str, idx64: bool = False) -> str:
+        """The group's code inside ``namespace ns`` for a batch module: grid sizes and interior
+        bounds are read from the per-grid ``Params`` and the LDS arrays are views of the
+        module's dynamic shared buffer ``sp_smem``, so sections share one LDS allocation."""
+        if not self.batched:
+            raise ValueError("section() needs a batched generator")
+        lines = [f"namespace {ns} {{"] + self._body(idx64) + ["} // namespace " + ns]
+        lines += [f"#undef {m}" for m in self._SECTION_MACROS]
+        lines += [f"#undef lds_s{i}" for i in self.lds_sources()]
+        return "\n".join(lines)
+
+    def lds_array_bytes(self) -> int:
+        s = self.sched
+        return sum(L_["D"] * L_["C"] * (s.ty + 2 * L_["H"] + 1) * (s.tx + 2 * L_["H"]) * 4
+                   for L_ in (self.src_layout[i] for i in self.lds_sources()))
+
+    def _body(self, idx64: bool = False) -> List[str]:
         s, g = self.sched, self.group
         n0, n1, n2 = self.grid
         total = n0 * n1 * n2 * max([L_["C"] for L_ in self.src_layout] + [9])
-        idx_t = "int" if total < 2 ** 31 - 1 else "i64"
+        idx_t = "i64" if (idx64 or total >= 2 ** 31 - 1) else "int"
         ns, no, nr, nd = len(g.sources), len(g.outputs), len(g.reductions), len(g.dscalars)
         ndet = len(self._det_inputs())
         lines = []
-        for hdr in ("sp_sym3.h", "fused_template.h"):
-            lines.append((CSRC / hdr).read_text().replace("#pragma once", ""))
-        lines.append(f"#define N0 {n0}\n#define N1 {n1}\n#define N2 {n2}\n"
+        if self.batched:
+            lines.append("#define N0 (p.n0)\n#define N1 (p.n1)\n#define N2 (p.n2)\n"
+                         f"#define TX {s.tx}\n#define TY {s.ty}\n#define ZC {s.zc}\n#define NT {s.threads}\n"
+                         f"#define HZ {self.hz}\n#define REDGE {self.redge}\n#define RP {self.rp}\n"
+                         f"#define VPT {self.v}\ntypedef {idx_t} IDX;\n"
+                         "#define FW 0\n#define NTB NT\n#define NBT 0\n#define TAIL_FIRST 0\n"
+                         "#define XLO_IN 0\n#define XHI_IN 0\n"
+                         "#define YLO (p.ylo)\n#define YHI (p.yhi)\n#define XLO (p.xlo)\n#define XHI (p.xhi)\n")
+        else:
+            lines.append(f"#define N0 {n0}\n#define N1 {n1}\n#define N2 {n2}\n"
                      f"#define TX {s.tx}\n#define TY {s.ty}\n#define ZC {s.zc}\n#define NT {s.threads}\n"
                      f"#define HZ {self.hz}\n#define REDGE {self.redge}\n#define RP {self.rp}\n"
                      f"#define VPT {self.v}\ntypedef {idx_t} IDX;\n"
                      f"#define FW {self.fw}\n#define NTB (NT + FW)\n"
+                     f"#define NBT {self.boundary_blocks()}\n#define TAIL_FIRST {int(TAIL_FIRST)}\n"
                      f"#define XLO_IN {int('lo' in self.xface_sides())}\n#define XHI_IN {int('hi' in self.xface_sides())}\n")
-        if self.use_lds:
+        if self.use_lds and not self.batched:
             _, _, ylo, yhi, xlo, xhi = self.interior_bounds()
             lines.append(f"#define YLO {ylo}\n#define YHI {yhi}\n#define XLO {xlo}\n#define XHI {xhi}\n")
         pf = [f"    const float* __restrict__ src{i};" for i in range(ns)]
@@ -1186,6 +1262,8 @@ class KernelGen:
         pf += [f"    double* dflag{i};" for i in range(ndet)]
         pf += [f"    const double* scal{i};" for i in range(nd)]
         pf.append("    int tiles_x, tiles_y, zchunks, nblocks;\n    float det_reg; int pad_;")
+        if self.batched:
+            pf.append("    int n0, n1, n2, ylo, yhi, xlo, xhi, nbt;\n    int bstart[8];\n    int boxes[36];")
         lines.append("struct Params {\n" + "\n".join(pf) + "\n};\n")
         acc = []
         for ri, nid in enumerate(g.reductions):
@@ -1196,28 +1274,41 @@ class KernelGen:
         acc += [f"    float sc{i};" for i in range(nd)]
         acc.append("    int dummy_;")
         lines.append("struct Acc {\n" + "\n".join(acc) + "\n};\n")
+        off = 0
         for i in self.lds_sources():
             L_ = self.src_layout[i]
-            # one spare row per slot absorbs the stores of unused halo entries
-            lines.append(f"__shared__ float lds_s{i}[{L_['D']}][{L_['C'] + 1}][TY + {2 * L_['H']}][TX + {2 * L_['H']}];")
+            dims = f"[{L_['D']}][{L_['C']}][TY + {2 * L_['H'] + 1}][TX + {2 * L_['H']}]"
+            if self.batched:
+                lines.append(f"#define lds_s{i} (*reinterpret_cast<float (*){dims}>(sp_smem + {off}))")
+                off += L_["D"] * L_["C"] * (s.ty + 2 * L_["H"] + 1) * (s.tx + 2 * L_["H"])
+            else:
+                # one spare row per channel (the last channel's absorbs the stores of unused
+                # halo entries)
+                lines.append(f"__shared__ float lds_s{i}{dims};")
         fns = []
         for m in self._variants():
             name, fn = self._emit_variant(m)
             fns.append((m, name))
             lines.append(fn)
         lines.append(self._kernel(fns, ndet))
-        return "\n".join(lines)
+        return lines
 
     # -- kernel skeleton ---------------------------------------------------------------------
     def _kernel(self, fns, ndet: int) -> str:
         s, g = self.sched, self.group
         out: List[str] = []
         A = out.append
-        A('extern "C" __global__ void __launch_bounds__(NTB) fused_kernel(const Params p) {')
-        A("    __shared__ float red_scratch[4 * NTB / 64 + 4];")
-        A("    const int tid = threadIdx.x;")
-        # XCD-aware block remap: XCD k (hardware block id % 8) walks a contiguous tile range
-        A("    const int b = blockIdx.x, nb = p.nblocks, per = nb >> 3, rem = nb & 7;")
+        if self.batched:
+            # one work unit (tile or tail block) of one grid; called from the batch kernel
+            A("__device__ __forceinline__ void run_unit(const Params& p, const int b, float* red_scratch) {")
+            A("    const int tid = threadIdx.x, rb = b;")
+            A("    const int nb = p.nblocks;")
+        else:
+            A('extern "C" __global__ void __launch_bounds__(NTB) fused_kernel(const Params p) {')
+            A("    __shared__ float red_scratch[4 * NTB / 64 + 4];")
+            A("    const int tid = threadIdx.x, rb = blockIdx.x;")
+            # XCD-aware block remap: XCD k (hardware block id % 8) walks a contiguous tile range
+            A("    const int b = blockIdx.x, nb = p.nblocks, per = nb >> 3, rem = nb & 7;")
         A("    Acc A;")
         for ri in range(len(g.reductions)):
             A(f"    A.r{ri}.init();")
@@ -1227,7 +1318,12 @@ class KernelGen:
             A(f"    A.sc{i} = (float)p.scal{i}[0];")
         if self.use_lds:
             self._boundary_part(A, ndet)
-        A("    const int xcd = b & 7, lb = xcd * per + (xcd < rem ? xcd : rem) + (b >> 3);")
+        if self.batched:
+            # the unit list is ordered on the host so that each XCD gets contiguous tiles
+            A("    const int lb = b;")
+        else:
+            A("    const int bm = TAIL_FIRST ? b - NBT : b;")
+            A("    const int xcd = bm & 7, lb = xcd * per + (xcd < rem ? xcd : rem) + (bm >> 3);")
         A("    const int tx_i = lb % p.tiles_x, t_ = lb / p.tiles_x;")
         A("    const int ty_i = t_ % p.tiles_y, zc_i = t_ / p.tiles_y;")
         A("    const int x0 = tx_i * TX, y0 = ty_i * TY;")
@@ -1255,11 +1351,11 @@ class KernelGen:
             # partial tiles are evaluated by the boundary kernel; write empty records
             A("    if (partial) {")
             for ri, nid in enumerate(g.reductions):
-                A(f"        if (tid == 0) {{ double* r_ = p.red{ri} + (i64)blockIdx.x * 4; r_[0] = 0.0; r_[1] = 0.0; "
+                A(f"        if (tid == 0) {{ double* r_ = p.red{ri} + (i64)rb * 4; r_[0] = 0.0; r_[1] = 0.0; "
                   f"r_[2] = {'INFINITY' if self.graph.node(nid).op_name == 'min_max' else '0.0'}; r_[3] = 0.0; "
                   f"{'r_[0] = INFINITY; r_[1] = -INFINITY;' if self.graph.node(nid).op_name == 'min_max' else ''} }}")
             for i in range(ndet):
-                A(f"        if (tid == 0) {{ double* r_ = p.dflag{i} + (i64)blockIdx.x * 4; r_[0] = INFINITY; r_[1] = -INFINITY; r_[2] = 0.0; r_[3] = 0.0; }}")
+                A(f"        if (tid == 0) {{ double* r_ = p.dflag{i} + (i64)rb * 4; r_[0] = INFINITY; r_[1] = -INFINITY; r_[2] = 0.0; r_[3] = 0.0; }}")
             A("        return;")
             A("    }")
         names = {m: n for m, n in fns}
@@ -1317,7 +1413,7 @@ class KernelGen:
                   f" const int q = e3 - (r - TY - {H}) * {W * C}; col = q / {C}; ch = q - col * {C}; }}")
                 A(f"        hg{i}[j] = (wrapm(y0 - {H} + r, N1) * N2 + wrapm(x0 - {H} + col, N2)) * {C} + ch;")
                 # out-of-range entries write a spare word at the end of the slot array
-                A(f"        hl{i}[j] = e < {HT} ? (ch * {RH} + r) * {W} + col : {C * RH * W};")
+                A(f"        hl{i}[j] = e < {HT} ? (ch * {RH + 1} + r) * {W} + col : {((C - 1) * (RH + 1) + RH) * W};")
                 A("    }")
             for buf in ("A", "B"):
                 A(f"    float {buf}o{i}[VPT][{C}];")
@@ -1349,7 +1445,7 @@ class KernelGen:
         A(f"{indent}  #pragma unroll")
         A(f"{indent}  for (int k = 0; k < VPT; ++k)")
         A(f"{indent}    #pragma unroll")
-        A(f"{indent}    for (int c = 0; c < {C}; ++c) ls[(c * {RH} + ly0 + k * RP + {H}) * {W} + lx + {H}] = {buf}o{i}[k][c];")
+        A(f"{indent}    for (int c = 0; c < {C}; ++c) ls[(c * {RH + 1} + ly0 + k * RP + {H}) * {W} + lx + {H}] = {buf}o{i}[k][c];")
         if KH:
             A(f"{indent}  #pragma unroll")
             A(f"{indent}  for (int j = 0; j < {KH}; ++j) ls[hl{i}[j]] = {buf}h{i}[j];")
@@ -1528,7 +1624,7 @@ class FusedExecutor:
             s = self.schedule_fn(group, graph)
             if s is not None:
                 return s
-        return DEFAULT_SCHEDULE
+        return heuristic_schedule(group.grid)
 
     def prepare(self, graph: Graph, values: Dict[str, Any], wanted: Sequence[str],
                 out_dtype: Optional[torch.dtype] = None, schedule: Optional[Schedule] = None):

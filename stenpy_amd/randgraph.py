@@ -14,14 +14,22 @@ from typing import List, Tuple
 UNARY_SAME = ["exp", "tanh", "clamp"]
 
 
-def random_expression(rng: random.Random, n_ops: int, want_mean: bool = None) -> Tuple[str, int]:
-    """Expression over field ``f`` (scalar) and ``u`` (vector); returns (expr, op count)."""
-    pool: List[Tuple[str, str]] = [("f", "S"), ("u", "V")]
+def random_expression(rng: random.Random, n_ops: int, want_mean: bool = None,
+                      vector_leaf: bool = True, mid_mean: float = 0.0) -> Tuple[str, int]:
+    """Expression over field ``f`` (scalar) and, with ``vector_leaf``, ``u`` (vector).
+
+    ``mid_mean`` is the probability that a scalar step adds the mean of another scalar
+    expression (a reduction feeding later pointwise work).  Returns (expr, op count)."""
+    pool: List[Tuple[str, str]] = [("f", "S"), ("u", "V")] if vector_leaf else [("f", "S")]
     ops = 0
     while ops < n_ops:
         e, t = rng.choice(pool)
         r = rng.random()
-        if t == "S":
+        if t == "S" and mid_mean and rng.random() < mid_mean:
+            e2 = rng.choice([p for p in pool if p[1] == "S"])[0]
+            cand = (f"({e} - mean({e2}))", "S")
+            ops += 1
+        elif t == "S":
             if r < 0.25:
                 cand = (f"gradient({e})", "V")
             elif r < 0.45:
@@ -32,7 +40,9 @@ def random_expression(rng: random.Random, n_ops: int, want_mean: bool = None) ->
                     cand = (f"exp(tanh({e}))", "S")
                     ops += 1
                 elif fn == "clamp":
-                    cand = (f"clamp({e}, lo=-1, hi=1)", "S")
+                    # keyword arguments are only encoded on calls without nested parentheses
+                    # (the reference front-end's rule), so bounds are given to bare symbols only
+                    cand = (f"clamp({e}, lo=-1, hi=1)", "S") if e.isidentifier() else (f"clamp({e})", "S")
                 else:
                     cand = (f"tanh({e})", "S")
             else:

@@ -28,11 +28,12 @@ _DB_LOCK = threading.Lock()
 # threads must cover whole tile rows: NT % TX == 0 and TY a multiple of NT / TX
 CANDIDATES: List[Schedule] = [
     Schedule(tx, ty, zc, nt)
-    for tx in (32, 64, 128) for ty in (4, 8, 16) for zc in (16, 32, 64, 128) for nt in (256, 512)
-    if nt % tx == 0 and ty % (nt // tx) == 0
+    for tx in (32, 64, 128) for ty in (4, 8, 16) for zc in (32, 64, 128, 256) for nt in (128, 256, 512)
+    if nt % tx == 0 and ty % (nt // tx) == 0 and 1 <= tx * ty // nt <= 4
 ]
-# a smaller grid of candidates for the training-data sweep
-DATA_CANDIDATES: List[Schedule] = [c for c in CANDIDATES if c.zc in (32, 64, 128)]
+# a smaller grid of candidates for the training-data sweep (the heuristic schedule of each
+# grid is always added)
+DATA_CANDIDATES: List[Schedule] = [c for c in CANDIDATES if c.zc in (64, 256) or (c.zc == 128 and c.tx == 64)]
 
 
 # ------------------------------------------------------------------------------ database
@@ -79,6 +80,9 @@ def group_signature(group, graph) -> str:
 
 
 _MODEL = None
+DATA_DIR = Path(__file__).resolve().parent / "data"
+MODEL_PATH = DATA_DIR / "schedule_mlp.npz"
+ROWS_PATH = DATA_DIR / "tune_rows.jsonl.gz"
 
 
 def install_model(model) -> None:
@@ -87,15 +91,27 @@ def install_model(model) -> None:
     _MODEL = model
 
 
+def load_default_model(device="cuda") -> Optional["ScheduleModel"]:
+    """The committed trained schedule model (None when absent)."""
+    global _MODEL
+    if _MODEL is None and MODEL_PATH.exists():
+        _MODEL = ScheduleModel.load(MODEL_PATH, device)
+    return _MODEL
+
+
 def schedule_fn(device: Optional[torch.device] = None, use_model: bool = True):
+    """Schedule source for the fused executor: tuning database, then the learned model, then
+    (by returning None) the executor's shape heuristic."""
     db = load_db()
 
     def pick(group, graph) -> Optional[Schedule]:
         hit = db.get(group_signature(group, graph))
         if hit:
             return Schedule(*hit)
-        if use_model and _MODEL is not None:
-            return _MODEL.best(group, graph)
+        if use_model:
+            m = load_default_model(device or "cuda")
+            if m is not None:
+                return m.best(group, graph)
         return None
 
     return pick
@@ -126,7 +142,8 @@ def group_stats(gen: F.KernelGen) -> Dict[str, float]:
     }
 
 
-def features(stats: Dict[str, float], grid: Sequence[int], s: Schedule, lds_bytes: int) -> List[float]:
+def features(stats: Dict[str, float], grid: Sequence[int], s: Schedule, lds_bytes: int,
+             face_wave: int = 0) -> List[float]:
     n0, n1, n2 = grid
     tiles = math.ceil(n2 / s.tx) * math.ceil(n1 / s.ty) * math.ceil(n0 / s.zc)
     h = stats["halo"]
@@ -139,22 +156,40 @@ def features(stats: Dict[str, float], grid: Sequence[int], s: Schedule, lds_byte
         math.log2(s.tx), math.log2(s.ty), math.log2(s.zc), s.threads / 256.0,
         lds_bytes / 1024.0, blocks_per_cu, math.log2(tiles), s.tx * s.ty / s.threads,
         halo_ratio, (2 * stats["hz"] + 1) / s.zc, tiles / 256.0,
-        (n2 % s.tx != 0) + (n1 % s.ty != 0), min(n0, s.zc) / s.zc,
+        (n2 % s.tx != 0) + (n1 % s.ty != 0), min(n0, s.zc) / s.zc, face_wave / 64.0,
     ]
 
 
-N_FEATURES = 28
+N_FEATURES = 29
 
 
 # ------------------------------------------------------------------------------ search
+_POOL = None
+
+
 def _compile_all(sources: List[str], workers: int) -> List[bytes]:
-    with cf.ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
-        return [c for c, _ in ex.map(F.compile_source, sources)]
+    """Compile in a pool of spawned processes (hipRTC serialises compiles within a process).
+
+    The workers only run hipRTC and never touch a device; each compiled code object also
+    lands in the on-disk cache, so the parent's later ``load_module`` is a cache hit."""
+    global _POOL
+    if workers <= 1 or len(sources) <= 1:
+        return [F.compile_source(s)[0] for s in sources]
+    if _POOL is None:
+        import multiprocessing as mp
+        import atexit
+        _POOL = cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn"))
+        atexit.register(_POOL.shutdown)
+    return [c for c, _ in _POOL.map(F.compile_source, sources)]
 
 
 def candidates_for(graph, group, info, out_dtypes, cands: Sequence[Schedule] = None):
     gens = []
-    for s in (cands or CANDIDATES):
+    cands = list(cands or CANDIDATES)
+    h = F.heuristic_schedule(group.grid)
+    if h not in cands:
+        cands.append(h)
+    for s in cands:
         try:
             gen = F.KernelGen(graph, group, info, s, out_dtypes, {})
         except ValueError:
@@ -165,7 +200,7 @@ def candidates_for(graph, group, info, out_dtypes, cands: Sequence[Schedule] = N
 
 
 def autotune(graph, runtime, wanted: Sequence[str], seed: Optional[Dict[str, Any]] = None,
-             cands: Sequence[Schedule] = None, reps: int = 10, workers: int = 16,
+             cands: Sequence[Schedule] = None, reps: int = 10, workers: int = 12,
              update_db: bool = True, out_dtype: Optional[torch.dtype] = None) -> List[Dict[str, Any]]:
     """Exhaustive schedule search for every fused group of ``graph``; returns timing rows."""
     ex = runtime.fused
@@ -192,7 +227,7 @@ def autotune(graph, runtime, wanted: Sequence[str], seed: Optional[Dict[str, Any
             nvox = int(np.prod(st.group.grid))
             rows.append({"sig": sig, "grid": list(st.group.grid), "schedule": list(gen.sched.as_tuple()),
                          "ms": ms, "ns_per_voxel": ms * 1e6 / nvox, "lds": gen.lds_bytes(),
-                         "stats": stats, "features": features(stats, st.group.grid, gen.sched, gen.lds_bytes())})
+                         "stats": stats, "features": features(stats, st.group.grid, gen.sched, gen.lds_bytes(), gen.fw)})
             if best is None or ms < best[0]:
                 best = (ms, gen.sched)
         if update_db and best is not None:
@@ -202,7 +237,11 @@ def autotune(graph, runtime, wanted: Sequence[str], seed: Optional[Dict[str, Any
     return rows
 
 
-def default_time(rows: List[Dict[str, Any]], sched: Schedule = F.DEFAULT_SCHEDULE) -> Optional[float]:
+def default_time(rows: List[Dict[str, Any]], sched: Optional[Schedule] = None) -> Optional[float]:
+    """Time of ``sched`` (default: the shape heuristic of the rows' grid) among ``rows``."""
+    if not rows:
+        return None
+    sched = sched or F.heuristic_schedule(rows[0]["grid"])
     for r in rows:
         if tuple(r["schedule"]) == sched.as_tuple():
             return r["ms"]
@@ -211,7 +250,7 @@ def default_time(rows: List[Dict[str, Any]], sched: Schedule = F.DEFAULT_SCHEDUL
 
 # ------------------------------------------------------------------------------ learned model
 def random_group_rows(n_graphs: int, seed: int, runtime, cands: Sequence[Schedule] = None,
-                      shape_range: Tuple[int, int] = (64, 256), reps: int = 5, workers: int = 16,
+                      shape_range: Tuple[int, int] = (64, 256), reps: int = 5, workers: int = 12,
                       log=None) -> List[Dict[str, Any]]:
     """Exhaustive timings of random config-3-style graphs on random grids (training data)."""
     import random
@@ -275,7 +314,7 @@ class ScheduleModel:
                 continue
             if gen.lds_bytes() > F.LDS_LIMIT:
                 continue
-            feats.append(features(stats, group.grid, s, gen.lds_bytes()))
+            feats.append(features(stats, group.grid, s, gen.lds_bytes(), gen.fw))
             scheds.append(s)
         if not scheds:
             return None
@@ -283,16 +322,28 @@ class ScheduleModel:
         return scheds[int(np.argmin(pred))]
 
     def save(self, path) -> None:
-        torch.save({"mlp": self.mlp.state_dict(), "cands": [c.as_tuple() for c in self.cands]}, path)
+        st = self.mlp.state_dict()
+        arrs = {"dims": np.asarray(st["dims"], dtype=np.int64),
+                "cands": np.asarray([c.as_tuple() for c in self.cands], dtype=np.int64),
+                "x_mean": st["x_mean"].numpy(), "x_std": st["x_std"].numpy(),
+                "y_norm": np.asarray([st["y_mean"], st["y_std"]], dtype=np.float64)}
+        for l, (w, b) in enumerate(zip(st["W"], st["b"])):
+            arrs[f"W{l}"] = w.numpy()
+            arrs[f"b{l}"] = b.numpy()
+        with open(path, "wb") as fh:
+            np.savez_compressed(fh, **arrs)
 
     @classmethod
     def load(cls, path, device="cuda") -> "ScheduleModel":
         from .mlp import ScheduleMLP
-        st = torch.load(path, map_location="cpu", weights_only=True)
-        dims = st["mlp"]["dims"]
+        z = np.load(path, allow_pickle=False)
+        dims = [int(v) for v in z["dims"]]
         mlp = ScheduleMLP(dims[0], dims[1], len(dims) - 1, device)
-        mlp.load_state_dict(st["mlp"])
-        return cls(mlp, [Schedule(*c) for c in st["cands"]])
+        mlp.load_state_dict({"W": [torch.from_numpy(z[f"W{l}"]) for l in range(len(dims) - 1)],
+                             "b": [torch.from_numpy(z[f"b{l}"]) for l in range(len(dims) - 1)],
+                             "x_mean": torch.from_numpy(z["x_mean"]), "x_std": torch.from_numpy(z["x_std"]),
+                             "y_mean": float(z["y_norm"][0]), "y_std": float(z["y_norm"][1])})
+        return cls(mlp, [Schedule(*(int(v) for v in c)) for c in z["cands"]])
 
 
 def evaluate_choice(rows: List[Dict[str, Any]], model: ScheduleModel) -> Dict[str, float]:
@@ -312,3 +363,119 @@ def evaluate_choice(rows: List[Dict[str, Any]], model: ScheduleModel) -> Dict[st
         hits += pick == best
     return {"groups": len(by), "mean_regret": float(np.mean(regrets)), "max_regret": float(np.max(regrets)),
             "top1": hits / max(len(by), 1), "geomean_gain_vs_default": float(np.exp(np.mean(np.log(gains)))) if gains else None}
+
+
+# ------------------------------------------------------------------------------ command line
+def _read_rows(path) -> List[Dict[str, Any]]:
+    import gzip
+    rows = []
+    with gzip.open(path, "rt") as fh:
+        for line in fh:
+            if line.strip():
+                rows.append(json.loads(line))
+    return rows
+
+
+def _append_rows(path, rows: List[Dict[str, Any]]) -> None:
+    import gzip
+    with gzip.open(path, "at") as fh:
+        for r in rows:
+            fh.write(json.dumps(r) + "\n")
+
+
+def split_by_graph(rows: List[Dict[str, Any]], holdout_every: int = 5):
+    train = [r for r in rows if r.get("graph", 0) % holdout_every != 0]
+    test = [r for r in rows if r.get("graph", 0) % holdout_every == 0]
+    return train, test
+
+
+def train_and_report(rows: List[Dict[str, Any]], device="cuda", epochs: int = 150,
+                     seed: int = 0) -> Tuple["ScheduleModel", Dict[str, Any]]:
+    """Fit on 4/5 of the graphs, report regret / gain on the held-out fifth."""
+    train, test = split_by_graph(rows)
+    model = ScheduleModel(cands=CANDIDATES)
+    losses = model.fit(train, device=device, epochs=epochs, seed=seed)
+    rep = {"rows_train": len(train), "rows_test": len(test), "epochs": epochs,
+           "loss_first": losses[0], "loss_last": losses[-1],
+           "train": evaluate_choice(train, model), "heldout": evaluate_choice(test, model)}
+    return model, rep
+
+
+def main(argv=None) -> int:
+    import argparse
+    import time
+    ap = argparse.ArgumentParser(prog="python -m stenpy_amd.tune")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("collect", help="exhaustive timings of random graphs (training rows)")
+    c.add_argument("--graphs", type=int, default=100)
+    c.add_argument("--first", type=int, default=0)
+    c.add_argument("--seed", type=int, default=9)
+    c.add_argument("--out", default=str(ROWS_PATH))
+    c.add_argument("--min-shape", type=int, default=48)
+    c.add_argument("--max-shape", type=int, default=256)
+    c.add_argument("--time-limit", type=float, default=1e9, help="stop after this many seconds")
+    c.add_argument("--workers", type=int, default=12)
+    t = sub.add_parser("train", help="train the schedule MLP on collected rows")
+    t.add_argument("--rows", default=str(ROWS_PATH))
+    t.add_argument("--out", default=str(MODEL_PATH))
+    t.add_argument("--report", default=str(DATA_DIR / "schedule_mlp_report.json"))
+    t.add_argument("--epochs", type=int, default=150)
+    u = sub.add_parser("tune", help="exhaustive search for one expression; stores the winner")
+    u.add_argument("--expr", required=True)
+    u.add_argument("--shape", required=True, help="N0xN1xN2")
+    u.add_argument("--boundary", default="periodic")
+    u.add_argument("--reductions", default="")
+    a = ap.parse_args(argv)
+
+    from .runtime import Runtime
+    dev = torch.device("cuda", 0)
+    if a.cmd == "collect":
+        import random
+        rt = Runtime(device=dev, fusion_enabled=True)
+        t0 = time.time()
+        Path(a.out).parent.mkdir(parents=True, exist_ok=True)
+        n_rows = 0
+        for gi in range(a.first, a.first + a.graphs):
+            if time.time() - t0 > a.time_limit:
+                print(f"time limit reached after graph {gi - 1}", flush=True)
+                break
+            rows = random_group_rows(1, a.seed * 100003 + gi, rt, cands=DATA_CANDIDATES,
+                                     shape_range=(a.min_shape, a.max_shape), workers=a.workers,
+                                     log=lambda m: print(m, flush=True))
+            for r in rows:
+                r["graph"] = gi
+            _append_rows(a.out, rows)
+            n_rows += len(rows)
+            rt.fused._plans.clear()
+            print(f"[{time.time() - t0:.0f}s] graph {gi}: {len(rows)} rows, total {n_rows}", flush=True)
+        return 0
+    if a.cmd == "train":
+        rows = _read_rows(a.rows)
+        model, rep = train_and_report(rows, device=dev, epochs=a.epochs)
+        model.save(a.out)
+        Path(a.report).write_text(json.dumps(rep, indent=1) + "\n")
+        print(json.dumps(rep), flush=True)
+        return 0
+    if a.cmd == "tune":
+        from . import gen
+        from .compiler import compile_expression
+        shape = tuple(int(v) for v in a.shape.lower().split("x"))
+        f = gen.generate(shape, 2, dev)
+        u_ = gen.generate_vector(shape, (3, 4, 5), dev)
+        g, sink, *_ = compile_expression(a.expr, boundary=a.boundary, field_map={"f": f, "u": u_})
+        reds = [g.add(r, (sink,), {}) for r in a.reductions.split(",") if r]
+        rt = Runtime(device=dev, fusion_enabled=True)
+        rows = autotune(g, rt, [sink] + reds)
+        by = {}
+        for r in rows:
+            by.setdefault(r["sig"], []).append(r)
+        for sig, rs in by.items():
+            rs.sort(key=lambda r: r["ms"])
+            print(json.dumps({"grid": rs[0]["grid"], "best": rs[0]["schedule"], "best_ms": rs[0]["ms"],
+                              "heuristic_ms": default_time(rs), "n": len(rs)}), flush=True)
+        return 0
+    return 2
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -160,3 +160,14 @@ def test_operator_dispatch_returns_pipeline():
     assert repr(p).startswith("Pipeline(gradient(f**2)")
     with pytest.raises(AttributeError):
         sp.no_such_operator
+
+
+def test_keyword_arguments_need_a_call_without_nested_parentheses():
+    """The front-end encodes ``k=v`` only inside innermost calls (engine 3525-3543): bounds on
+    a bare symbol parse, bounds on a nested call are rejected as unknown sympify options."""
+    f = torch.zeros(4, 4, 4, dtype=torch.float64)
+    g, sink, *_ = sp.compile_expression("clamp(f, lo=-1, hi=1)", field_map={"f": f})
+    node = g.node(sink)
+    assert node.op_name == "clamp" and node.params["lo"] == -1 and node.params["hi"] == 1
+    with pytest.raises(ValueError):
+        sp.compile_expression("clamp(laplacian(f), lo=-1, hi=1)", field_map={"f": f})
