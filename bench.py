@@ -79,6 +79,30 @@ def cpu_baseline(n: int, planes: int):
     return vox / ts[1], threads, f"{planes}x{n}x{n} slab, FP64 torch CPU oracle, median of 3"
 
 
+def device_copy_gbs(dev) -> float:
+    """Read+write GB/s of a 2 GiB device-to-device copy on this box (a reference for the
+    HBM rate this particular GPU sustains; it varies between boxes)."""
+    import torch
+    a = torch.empty(1 << 28, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    return round(2 * a.numel() * 4 * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+
+
+def schedule_source(cg, g) -> str:
+    from stenpy_amd import tune
+    if tune.load_db().get(tune.group_signature(cg.group, g)):
+        return "autotuned (exhaustive search, tuning database)"
+    return "learned model or shape heuristic"
+
+
 def _line(metric, value, unit, steps, warmup, ms, config, **extra):
     d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
          "warmup": warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -315,6 +339,7 @@ def main() -> int:
 
     cg, ln = group_launch(rt, g, res, sink)
     k_ms = ln.time_ms(reps=max(10, args.steps))
+    copy_gbs = device_copy_gbs(dev)
     alg_bytes = 20 * nvox
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
     traffic = None
@@ -339,6 +364,8 @@ def main() -> int:
                    "plain_ms_per_step": round(t_plain * 1e3, 3),
                    "speedup_vs_plain": round(t_plain / t_fused, 2),
                    "bf16_out_gvoxel_s": round(nvox / t_bf16 / 1e9, 2),
+                   "schedule_source": schedule_source(cg, g),
+                   "device_copy_gbs": copy_gbs,
                    "bf16_out_hbm_frac_step": round(18 * nvox / t_bf16 / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -98,8 +98,8 @@ def heuristic_schedule(grid: Sequence[int]) -> Schedule:
     while zc > 32 and (zc >= n0 or tiles * -(-n0 // zc) < 512):
         zc //= 2
     return Schedule(tx, ty, zc, nt)
-# LDS per block kept within 64 KiB so that at least two blocks share a CU
-LDS_LIMIT = 64 * 1024
+# LDS per block kept within 80 KiB so that at least two blocks share a CU (160 KiB per CU)
+LDS_LIMIT = 80 * 1024
 FALLBACK_SCHEDULES = [Schedule(64, 4, 64, 256), Schedule(32, 8, 64, 256), Schedule(32, 4, 64, 128),
                       Schedule(16, 8, 64, 128), Schedule(16, 4, 64, 64)]
 # Edge handling of the main kernel.  "band": only voxels at least the edge reach away from
@@ -107,9 +107,11 @@ FALLBACK_SCHEDULES = [Schedule(64, 4, 64, 256), Schedule(32, 8, 64, 256), Schedu
 # face band goes to the gather kernel.  "variants": tiles touching an x/y face run
 # edge-specialised LDS variants.  "auto" picks "band" when the band is a small share.
 EDGE_MODE = os.environ.get("STENPY_EDGE_MODE", "auto")
-# Tail blocks (gather evaluation of the face band) at the start of the grid instead of the end
-# (measured slower on the 512^3 graph; kept as an option).
-TAIL_FIRST = os.environ.get("STENPY_TAIL_FIRST", "0") == "1"
+# The gather evaluation of the face band (z faces, y faces, corners, partial tiles) runs as a
+# separate small kernel (no LDS arrays, few registers) on a side stream, beside the LDS
+# z-marching kernel; off: the same work as extra blocks at the end of the main grid.
+SPLIT_TAIL = os.environ.get("STENPY_SPLIT_TAIL", "1") == "1"
+EXTRA_LDS = int(os.environ.get("STENPY_EXTRA_LDS", "0"))
 BAND_MAX_FRACTION = 0.08
 
 
@@ -1062,18 +1064,18 @@ class KernelGen:
 
     def boundary_blocks(self, grid=None) -> int:
         tot = sum((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]) for b in self.boundary_boxes(grid))
-        n = (tot + self.block_threads() - 1) // self.block_threads()
-        if self.batched:
-            return n
-        # a multiple of 8 keeps the main blocks' hardware ids congruent to their XCD index
-        return (n + 7) // 8 * 8 if (TAIL_FIRST and n) else n
+        return (tot + self.block_threads() - 1) // self.block_threads()
+
+    @property
+    def split_tail(self) -> bool:
+        return SPLIT_TAIL and self.use_lds and not self.batched and self.boundary_blocks() > 0
 
     def _boundary_part(self, A, ndet: int) -> None:
-        """Tail blocks (blockIdx.x >= p.nblocks): gather evaluation of the boundary boxes."""
+        """Tail blocks (block index b >= p.nblocks): gather evaluation of the boundary boxes."""
         g = self.group
         boxes = self.boundary_boxes()
-        A("    if (TAIL_FIRST ? b < NBT : b >= nb) {")
-        A("        const i64 t = (i64)(TAIL_FIRST ? b : b - nb) * NTB + tid;")
+        A("    if (b >= nb) {")
+        A("        const i64 t = (i64)(b - nb) * NTB + tid;")
         A("        int z = -1, y = 0, x = 0;")
         if self.batched:
             # boxes of this grid, from the parameters: bstart = prefix sums of box sizes
@@ -1202,7 +1204,7 @@ class KernelGen:
 
     # macros a generated section defines (undefined again at the end of a batched section)
     _SECTION_MACROS = ("N0", "N1", "N2", "TX", "TY", "ZC", "NT", "HZ", "REDGE", "RP", "VPT", "FW", "NTB",
-                       "NBT", "TAIL_FIRST", "XLO_IN", "XHI_IN", "YLO", "YHI", "XLO", "XHI")
+                       "XLO_IN", "XHI_IN", "YLO", "YHI", "XLO", "XHI")
 
     def source(self) -> str:
         if self.batched:
@@ -1242,7 +1244,7 @@ class KernelGen:
                          f"#define TX {s.tx}\n#define TY {s.ty}\n#define ZC {s.zc}\n#define NT {s.threads}\n"
                          f"#define HZ {self.hz}\n#define REDGE {self.redge}\n#define RP {self.rp}\n"
                          f"#define VPT {self.v}\ntypedef {idx_t} IDX;\n"
-                         "#define FW 0\n#define NTB NT\n#define NBT 0\n#define TAIL_FIRST 0\n"
+                         "#define FW 0\n#define NTB NT\n"
                          "#define XLO_IN 0\n#define XHI_IN 0\n"
                          "#define YLO (p.ylo)\n#define YHI (p.yhi)\n#define XLO (p.xlo)\n#define XHI (p.xhi)\n")
         else:
@@ -1251,7 +1253,7 @@ class KernelGen:
                      f"#define HZ {self.hz}\n#define REDGE {self.redge}\n#define RP {self.rp}\n"
                      f"#define VPT {self.v}\ntypedef {idx_t} IDX;\n"
                      f"#define FW {self.fw}\n#define NTB (NT + FW)\n"
-                     f"#define NBT {self.boundary_blocks()}\n#define TAIL_FIRST {int(TAIL_FIRST)}\n"
+
                      f"#define XLO_IN {int('lo' in self.xface_sides())}\n#define XHI_IN {int('hi' in self.xface_sides())}\n")
         if self.use_lds and not self.batched:
             _, _, ylo, yhi, xlo, xhi = self.interior_bounds()
@@ -1291,7 +1293,29 @@ class KernelGen:
             fns.append((m, name))
             lines.append(fn)
         lines.append(self._kernel(fns, ndet))
+        if self.split_tail:
+            lines.append(self._tail_kernel(ndet))
         return lines
+
+    def _tail_kernel(self, ndet: int) -> str:
+        """``fused_tail``: the boundary boxes alone, block b = p.nblocks + blockIdx.x (its
+        records follow the main kernel's)."""
+        g = self.group
+        out: List[str] = []
+        A = out.append
+        A('extern "C" __global__ void __launch_bounds__(NTB) fused_tail(const Params p) {')
+        A("    __shared__ float red_scratch[4 * NTB / 64 + 4];")
+        A("    const int tid = threadIdx.x, nb = p.nblocks, b = nb + (int)blockIdx.x, rb = b;")
+        A("    Acc A;")
+        for ri in range(len(g.reductions)):
+            A(f"    A.r{ri}.init();")
+        for i in range(ndet):
+            A(f"    A.dflag{i} = 0.0f;")
+        for i in range(len(g.dscalars)):
+            A(f"    A.sc{i} = (float)p.scal{i}[0];")
+        self._boundary_part(A, ndet)
+        A("}")
+        return "\n".join(out)
 
     # -- kernel skeleton ---------------------------------------------------------------------
     def _kernel(self, fns, ndet: int) -> str:
@@ -1316,14 +1340,13 @@ class KernelGen:
             A(f"    A.dflag{i} = 0.0f;")
         for i in range(len(g.dscalars)):
             A(f"    A.sc{i} = (float)p.scal{i}[0];")
-        if self.use_lds:
+        if self.use_lds and not self.split_tail:
             self._boundary_part(A, ndet)
         if self.batched:
             # the unit list is ordered on the host so that each XCD gets contiguous tiles
             A("    const int lb = b;")
         else:
-            A("    const int bm = TAIL_FIRST ? b - NBT : b;")
-            A("    const int xcd = bm & 7, lb = xcd * per + (xcd < rem ? xcd : rem) + (bm >> 3);")
+            A("    const int xcd = b & 7, lb = xcd * per + (xcd < rem ? xcd : rem) + (b >> 3);")
         A("    const int tx_i = lb % p.tiles_x, t_ = lb / p.tiles_x;")
         A("    const int ty_i = t_ % p.tiles_y, zc_i = t_ / p.tiles_y;")
         A("    const int x0 = tx_i * TX, y0 = ty_i * TY;")
@@ -1587,6 +1610,13 @@ class CompiledGroup:
     gen: KernelGen
     source: str
     module: Optional[L.Module] = None
+    tail: Optional[L.Module] = None
+
+    def tail_module(self) -> L.Module:
+        if self.tail is None:
+            code, _ = compile_source(self.source)
+            self.tail = L.Module(code, "fused_tail")
+        return self.tail
 
     @property
     def group(self) -> Group:
@@ -1616,8 +1646,15 @@ class FusedExecutor:
         self.device = device
         self.schedule_fn = schedule_fn
         self.edge_mode = edge_mode
+        self._side: Optional[torch.cuda.Stream] = None
         self._plans: Dict[str, Tuple[List[Step], Dict[str, Info], Dict[int, CompiledGroup]]] = {}
         self.last_timing: Dict[str, float] = {}
+
+    def side_stream(self) -> "torch.cuda.Stream":
+        """Stream of the tail kernels (default priority, created once)."""
+        if self._side is None:
+            self._side = torch.cuda.Stream(device=self.device)
+        return self._side
 
     def schedule_for(self, group: Group, graph: Graph) -> Schedule:
         if self.schedule_fn is not None:
@@ -1724,7 +1761,8 @@ class FusedExecutor:
             if not isinstance(t, torch.Tensor):
                 t = torch.tensor(float(t))
             scal.append(t.to(device=dev, dtype=torch.float64).reshape(1).contiguous())
-        return Launch(cg, srcs, outs, reds, dflags, scal, nb)
+        side = self.side_stream() if cg.gen.split_tail else None
+        return Launch(cg, srcs, outs, reds, dflags, scal, nb, side)
 
     def _run_group(self, cg: CompiledGroup, graph: Graph, results: Dict[str, Any]) -> None:
         g = cg.group
@@ -1763,15 +1801,27 @@ class Launch:
     dflags: List[torch.Tensor]
     scal: List[torch.Tensor]
     nblocks: int
+    side: Optional["torch.cuda.Stream"] = None
 
     def args(self, det_reg: float = 0.0) -> bytes:
         return pack_params(self.cg, self.srcs, self.outs, self.reds, self.dflags, self.scal, det_reg)
 
     def launch(self, stream: Optional[int] = None, det_reg: float = 0.0) -> None:
-        st = stream if stream is not None else L.stream_ptr()
+        """Main kernel on ``stream`` (default: torch's current stream); the tail kernel, when
+        split, on the side stream between a fork and a join with ``stream``."""
+        gen = self.cg.gen
+        dev = self.srcs[0].device if self.srcs else self.outs[0].device
+        st = stream if stream is not None else L.stream_ptr(dev)
         a = self.args(det_reg)
-        grid = self.nblocks + self.cg.gen.boundary_blocks()
-        self.cg.module.launch((grid, 1, 1), (self.cg.gen.block_threads(), 1, 1), a, 0, st)
+        nbt = gen.boundary_blocks()
+        if self.side is None or not nbt:
+            self.cg.module.launch((self.nblocks + nbt, 1, 1), (gen.block_threads(), 1, 1), a, 0, st)
+            return
+        main = torch.cuda.current_stream(dev) if stream is None else torch.cuda.ExternalStream(st, device=dev)
+        self.side.wait_stream(main)
+        self.cg.module.launch((self.nblocks, 1, 1), (gen.block_threads(), 1, 1), a, EXTRA_LDS, st)
+        self.cg.tail_module().launch((nbt, 1, 1), (gen.block_threads(), 1, 1), a, 0, self.side.cuda_stream)
+        main.wait_stream(self.side)
 
     def time_ms(self, reps: int = 20, warmup: int = 3, rounds: int = 3) -> float:
         """Min over ``rounds`` of the mean kernel time, HIP events on the launch stream."""

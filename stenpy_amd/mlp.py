@@ -136,7 +136,8 @@ class ScheduleMLP:
 
     # -- training / inference on raw features ------------------------------------------------
     def fit(self, X: np.ndarray, y: np.ndarray, epochs: int = 200, batch: int = 512,
-            lr: float = 2e-3, seed: int = 0, verbose: bool = False) -> List[float]:
+            lr: float = 2e-3, seed: int = 0, verbose: bool = False,
+            weight_decay: float = 0.0) -> List[float]:
         X = torch.as_tensor(X, dtype=torch.float32)
         y = torch.as_tensor(y, dtype=torch.float32)
         self.x_mean, self.x_std = X.mean(0), X.std(0).clamp(min=1e-6)
@@ -157,7 +158,8 @@ class ScheduleMLP:
                 xb = Xd.index_select(0, idx).contiguous()
                 self.forward(xb)
                 loss = self.backward(yd.index_select(0, idx))
-                self.adam_step(lr * 0.5 * (1 + math.cos(math.pi * ep / epochs)))
+                self.adam_step(lr * 0.5 * (1 + math.cos(math.pi * ep / epochs)),
+                               weight_decay=weight_decay)
                 tot += loss
                 nb += 1
             losses.append(float(tot.item()) / nb)

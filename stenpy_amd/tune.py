@@ -202,7 +202,11 @@ def candidates_for(graph, group, info, out_dtypes, cands: Sequence[Schedule] = N
 def autotune(graph, runtime, wanted: Sequence[str], seed: Optional[Dict[str, Any]] = None,
              cands: Sequence[Schedule] = None, reps: int = 10, workers: int = 12,
              update_db: bool = True, out_dtype: Optional[torch.dtype] = None) -> List[Dict[str, Any]]:
-    """Exhaustive schedule search for every fused group of ``graph``; returns timing rows."""
+    """Schedule search for every fused group of ``graph``; returns timing rows.
+
+    ``cands``: candidate list (default: every valid candidate, i.e. exhaustive search), or a
+    callable (group, graph) -> candidates, e.g. a trained model's shortlist.  The shape
+    heuristic's schedule is always timed too."""
     ex = runtime.fused
     results = ex.run(graph, runtime, seed, wanted=wanted, out_dtype=out_dtype)   # inputs per group
     values = {n: results[n] for n in graph._order if graph.node(n).op_name == "_constant"}
@@ -213,7 +217,8 @@ def autotune(graph, runtime, wanted: Sequence[str], seed: Optional[Dict[str, Any
         if st.kind != "group":
             continue
         base = comp[si]
-        gens = candidates_for(graph, st.group, info, base.gen.out_dtypes, cands)
+        group_cands = cands(st.group, graph) if callable(cands) else cands
+        gens = candidates_for(graph, st.group, info, base.gen.out_dtypes, group_cands)
         srcs = [g.source() for g in gens]
         codes = _compile_all(srcs, workers)
         stats = group_stats(base.gen)
@@ -282,24 +287,40 @@ def random_group_rows(n_graphs: int, seed: int, runtime, cands: Sequence[Schedul
 
 
 class ScheduleModel:
-    """MLP over (group, schedule) features predicting log(ns per voxel); picks the argmin."""
+    """MLP over (group, schedule) features predicting a group-relative log time.
+
+    Only the ranking of a group's candidates matters, so the target is log(time) minus the
+    mean over that group's timed candidates: the network spends no capacity on how fast the
+    group is overall.  ``best`` returns the argmin; ``shortlist`` the k best predictions, which
+    the autotuner times (a k-candidate search instead of the exhaustive one)."""
 
     def __init__(self, mlp=None, cands: Sequence[Schedule] = None):
         self.mlp = mlp
         self.cands = list(cands or CANDIDATES)
 
-    def fit(self, rows: List[Dict[str, Any]], device="cuda", epochs: int = 150, seed: int = 0):
+    def fit(self, rows: List[Dict[str, Any]], device="cuda", epochs: int = 40, seed: int = 0,
+            weight_decay: float = 1e-3):
         from .mlp import ScheduleMLP
         X = np.asarray([r["features"] for r in rows], dtype=np.float32)
-        y = np.log(np.asarray([r["ns_per_voxel"] for r in rows], dtype=np.float32))
+        y = np.log(np.asarray([r["ns_per_voxel"] for r in rows], dtype=np.float64))
+        groups: Dict[Tuple, List[int]] = {}
+        for i, r in enumerate(rows):
+            groups.setdefault((r.get("graph"), r["sig"]), []).append(i)
+        for idx in groups.values():
+            y[idx] -= y[idx].mean()
         self.mlp = ScheduleMLP(X.shape[1], 256, 4, device, seed)
-        return self.mlp.fit(X, y, epochs=epochs, batch=256, lr=2e-3, seed=seed)
+        return self.mlp.fit(X, y.astype(np.float32), epochs=epochs, batch=256, lr=2e-3, seed=seed,
+                            weight_decay=weight_decay)
 
     def predict_rows(self, rows: List[Dict[str, Any]]) -> np.ndarray:
         X = np.asarray([r["features"] for r in rows], dtype=np.float32)
         return self.mlp.predict(X)
 
     def best(self, group, graph) -> Optional[Schedule]:
+        s = self.shortlist(group, graph, 1)
+        return s[0] if s else None
+
+    def shortlist(self, group, graph, k: int = 3) -> List[Schedule]:
         info = getattr(group, "_info", None)
         try:
             base = F.KernelGen(graph, group, info, F.DEFAULT_SCHEDULE, {}, {})
@@ -317,9 +338,9 @@ class ScheduleModel:
             feats.append(features(stats, group.grid, s, gen.lds_bytes(), gen.fw))
             scheds.append(s)
         if not scheds:
-            return None
-        pred = self.mlp.predict(np.asarray(feats, dtype=np.float32))
-        return scheds[int(np.argmin(pred))]
+            return []
+        pred = self.mlp.predict(np.asarray(feats, dtype=np.float32)).reshape(-1)
+        return [scheds[i] for i in np.argsort(pred)[:k]]
 
     def save(self, path) -> None:
         st = self.mlp.state_dict()
@@ -346,22 +367,23 @@ class ScheduleModel:
         return cls(mlp, [Schedule(*(int(v) for v in c)) for c in z["cands"]])
 
 
-def evaluate_choice(rows: List[Dict[str, Any]], model: ScheduleModel) -> Dict[str, float]:
-    """Per group: time of the model's pick vs the exhaustive best and vs the default."""
+def evaluate_choice(rows: List[Dict[str, Any]], model: ScheduleModel, k: int = 1) -> Dict[str, float]:
+    """Per group: time of the model's pick (the best of its ``k`` top predictions, as the
+    shortlist search times them) vs the exhaustive best and vs the shape heuristic."""
     by: Dict[Tuple, List[Dict[str, Any]]] = {}
     for r in rows:
         by.setdefault((r.get("graph"), r["sig"]), []).append(r)
     regrets, gains, hits = [], [], 0
     for key, rs in by.items():
-        pred = model.predict_rows(rs)
-        pick = rs[int(np.argmin(pred))]["ms"]
+        pred = np.asarray(model.predict_rows(rs)).reshape(-1)
+        pick = min(rs[i]["ms"] for i in np.argsort(pred)[:k])
         best = min(r["ms"] for r in rs)
         dflt = default_time(rs)
         regrets.append(pick / best - 1.0)
         if dflt is not None:
             gains.append(dflt / pick)
         hits += pick == best
-    return {"groups": len(by), "mean_regret": float(np.mean(regrets)), "max_regret": float(np.max(regrets)),
+    return {"k": k, "groups": len(by), "mean_regret": float(np.mean(regrets)), "max_regret": float(np.max(regrets)),
             "top1": hits / max(len(by), 1), "geomean_gain_vs_default": float(np.exp(np.mean(np.log(gains)))) if gains else None}
 
 
@@ -389,15 +411,18 @@ def split_by_graph(rows: List[Dict[str, Any]], holdout_every: int = 5):
     return train, test
 
 
-def train_and_report(rows: List[Dict[str, Any]], device="cuda", epochs: int = 150,
+def train_and_report(rows: List[Dict[str, Any]], device="cuda", epochs: int = 40,
                      seed: int = 0) -> Tuple["ScheduleModel", Dict[str, Any]]:
-    """Fit on 4/5 of the graphs, report regret / gain on the held-out fifth."""
+    """Fit on 4/5 of the graphs, report regret / gain on the held-out fifth (pick = the top
+    prediction, and the best of the top three as the shortlist search would time them)."""
     train, test = split_by_graph(rows)
     model = ScheduleModel(cands=CANDIDATES)
     losses = model.fit(train, device=device, epochs=epochs, seed=seed)
     rep = {"rows_train": len(train), "rows_test": len(test), "epochs": epochs,
+           "graphs": len({r.get("graph") for r in rows}),
            "loss_first": losses[0], "loss_last": losses[-1],
-           "train": evaluate_choice(train, model), "heldout": evaluate_choice(test, model)}
+           "train": evaluate_choice(train, model), "heldout": evaluate_choice(test, model),
+           "heldout_top3": evaluate_choice(test, model, k=3)}
     return model, rep
 
 
@@ -419,7 +444,7 @@ def main(argv=None) -> int:
     t.add_argument("--rows", default=str(ROWS_PATH))
     t.add_argument("--out", default=str(MODEL_PATH))
     t.add_argument("--report", default=str(DATA_DIR / "schedule_mlp_report.json"))
-    t.add_argument("--epochs", type=int, default=150)
+    t.add_argument("--epochs", type=int, default=40)
     u = sub.add_parser("tune", help="exhaustive search for one expression; stores the winner")
     u.add_argument("--expr", required=True)
     u.add_argument("--shape", required=True, help="N0xN1xN2")

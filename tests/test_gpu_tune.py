@@ -43,14 +43,16 @@ def test_autotune_records_the_fastest_candidate(tmp_path, monkeypatch):
 @pytest.mark.skipif(not tune.ROWS_PATH.exists(), reason="no committed timing rows")
 def test_learned_schedules_beat_the_heuristic_on_heldout_graphs():
     rows = tune._read_rows(tune.ROWS_PATH)
-    model, rep = tune.train_and_report(rows, device="cuda", epochs=120)
+    model, rep = tune.train_and_report(rows, device="cuda", epochs=40)
     print(json.dumps(rep))
-    held = rep["heldout"]
+    held, top3 = rep["heldout"], rep["heldout_top3"]
     assert held["groups"] >= 5
     assert rep["loss_last"] < 0.5 * rep["loss_first"]
-    # the network's pick is on average faster than the heuristic's and close to exhaustive
-    assert held["geomean_gain_vs_default"] > 1.0, held
-    assert held["mean_regret"] < 0.25, held
+    # the network's pick is on average faster than the heuristic's and close to exhaustive;
+    # timing its three best proposals comes closer still
+    assert held["geomean_gain_vs_default"] > 1.05, held
+    assert held["mean_regret"] < 0.2, held
+    assert top3["mean_regret"] < 0.08 and top3["mean_regret"] <= held["mean_regret"], top3
 
 
 @pytest.mark.skipif(not tune.MODEL_PATH.exists(), reason="no committed model")
@@ -67,3 +69,19 @@ def test_committed_model_loads_and_proposes_valid_schedules():
     assert s is not None and s in model.cands
     gen_ = F.KernelGen(g, steps[0].group, info, s, {}, {})
     assert gen_.lds_bytes() <= F.LDS_LIMIT
+
+
+@pytest.mark.skipif(not tune.MODEL_PATH.exists(), reason="no committed model")
+def test_shortlist_search_times_only_the_proposals(tmp_path, monkeypatch):
+    monkeypatch.setenv("STENPY_TUNE_DB", str(tmp_path / "db.json"))
+    model = tune.ScheduleModel.load(tune.MODEL_PATH, "cuda")
+    shape = (48, 64, 160)
+    f = gen.generate(shape, 9, "cuda")
+    g, sink, *_ = sp.compile_expression("laplacian(tanh(f)) * f", boundary="neumann", field_map={"f": f})
+    rt = sp.Runtime(device="cuda", fusion_enabled=True)
+    rows = tune.autotune(g, rt, [sink], cands=lambda grp, graph: model.shortlist(grp, graph, 3),
+                         reps=5, workers=1)
+    scheds = {tuple(r["schedule"]) for r in rows}
+    assert 1 <= len(scheds) <= 4            # three proposals (+ the heuristic when not among them)
+    db = json.loads((tmp_path / "db.json").read_text())
+    assert db[rows[0]["sig"]] == min(rows, key=lambda r: r["ms"])["schedule"]
