@@ -52,13 +52,16 @@ def structure_key(group, graph: Graph) -> str:
     return sig.split("|", 1)[1] if "|" in sig else sig
 
 
-def xcd_order(n: int) -> np.ndarray:
-    """Position -> logical unit: position w runs on XCD w % 8 (grid a multiple of 8, units
-    dealt round-robin), and each XCD walks a contiguous range of logical units."""
-    per, rem = n // 8, n % 8
-    w = np.arange(n)
-    x, j = w % 8, w // 8
-    return x * per + np.minimum(x, rem) + j
+def queue_split(costs: np.ndarray, n_queues: int = 8) -> np.ndarray:
+    """Boundaries [0, b1, ..., n] splitting units (in their order) into ``n_queues``
+    contiguous runs of about equal summed cost."""
+    c = np.cumsum(np.asarray(costs, dtype=np.float64))
+    tot = c[-1] if len(c) else 0.0
+    cuts = [0] + [int(np.searchsorted(c, tot * q / n_queues, side="right")) for q in range(1, n_queues)]
+    cuts = [min(max(v, 0), len(c)) for v in cuts] + [len(c)]
+    for i in range(1, len(cuts)):
+        cuts[i] = max(cuts[i], cuts[i - 1])
+    return np.asarray(cuts, dtype=np.int32)
 
 
 @dataclass
@@ -190,19 +193,42 @@ class BatchRunner:
                               "module": None})
 
     def _batch_kernel(self, k: int) -> str:
+        """Persistent kernel over 8 work queues (contiguous unit ranges of equal estimated
+        cost).  A block starts on queue blockIdx % 8 -- its XCD's, so neighbouring tiles share
+        that XCD's L2 -- and then steals from the others; each block scans every queue once,
+        so every block exits.  The last block to finish resets the counters for the next
+        launch (no memset, so the launch can be captured and replayed)."""
         nt = self.threads
-        lines = ["struct BatchArgs {", "    const int4* units;", "    int nunits, pad_;",
+        lines = ["struct BatchArgs {", "    const int4* units;", "    const int* qstart;", "    int* counters;",
                  f"    const void* params[{k}];", "};",
                  f'extern "C" __global__ void __launch_bounds__({nt}) batch_kernel(const BatchArgs a) {{',
                  f"    __shared__ float red_scratch[4 * {nt} / 64 + 4];",
-                 "    for (int w = blockIdx.x; w < a.nunits; w += gridDim.x) {",
-                 "        const int4 u = a.units[w];",
-                 "        switch (u.x) {"]
+                 "    __shared__ int s_w;",
+                 "    const int xq = blockIdx.x & 7;",
+                 "    for (int qi = 0; qi < 8; ++qi) {",
+                 "        const int q = (xq + qi) & 7, q0 = a.qstart[q], q1 = a.qstart[q + 1];",
+                 "        while (true) {",
+                 "            if (threadIdx.x == 0) s_w = q0 + atomicAdd(a.counters + q, 1);",
+                 "            __syncthreads();",
+                 "            const int w = s_w;",
+                 "            __syncthreads();   // every thread has read s_w",
+                 "            if (w >= q1) break;",
+                 "            const int4 u = a.units[w];",
+                 "            switch (u.x) {"]
         for i in range(k):
-            lines.append(f"        case {i}: g{i}::run_unit(reinterpret_cast<const g{i}::Params*>(a.params[{i}])[u.y],"
-                         " u.z, red_scratch); break;")
-        lines += ["        default: break;", "        }",
-                  "        __syncthreads();   // the next unit reuses the LDS", "    }", "}"]
+            # a local copy: no output store can alias it, so its fields stay in registers
+            lines.append(f"            case {i}: {{ const g{i}::Params prm = reinterpret_cast<const g{i}::Params*>(a.params[{i}])[u.y];"
+                         f" g{i}::run_unit(prm, u.z, red_scratch); }} break;")
+        lines += ["            default: break;", "            }",
+                  "            __syncthreads();   // the next unit reuses the LDS", "        }", "    }",
+                  "    if (threadIdx.x == 0) {",
+                  "        __threadfence();",
+                  "        if (atomicAdd(a.counters + 8, 1) == (int)gridDim.x - 1) {",
+                  "            for (int q = 0; q < 9; ++q) atomicExch(a.counters + q, 0);",
+                  "            __threadfence();",
+                  "        }",
+                  "    }",
+                  "}"]
         return "\n".join(lines)
 
     # -- buffers and parameters ------------------------------------------------------------------
@@ -283,13 +309,22 @@ class BatchRunner:
             for ki, es in enumerate(per_key):
                 blob = b"".join(self._params_bytes(e, mod["gens"][ki]) for e in es)
                 params.append(torch.frombuffer(bytearray(blob), dtype=torch.uint8).to(dev))
-            units = []
+            units, costs = [], []
             for ki, es in enumerate(per_key):
+                gen = mod["gens"][ki]
+                s = gen.sched
+                w = self._cost_weight(gen)
                 for ii, e in enumerate(es):
+                    n0 = e.group.grid[0]
+                    tile = s.tx * s.ty * min(s.zc, n0)
+                    tail = self.threads * gen.TAIL_VPT
                     units += [(ki, ii, b, 0) for b in range(e.nb + e.nbt)]
-            order = xcd_order(len(units))
-            u = np.asarray(units, dtype=np.int32).reshape(-1, 4)[order]
+                    costs += [w * tile] * e.nb + [w * tail] * e.nbt
+            u = np.asarray(units, dtype=np.int32).reshape(-1, 4)
+            qstart = queue_split(np.asarray(costs), 8)
             units_t = torch.from_numpy(np.ascontiguousarray(u)).to(dev)
+            qstart_t = torch.from_numpy(qstart).to(dev)
+            counters = torch.zeros(9, dtype=torch.int32, device=dev)
             descs = []
             for e in entries:
                 graph = self.items[e.item][0]
@@ -302,11 +337,19 @@ class BatchRunner:
                                              cnt, F.RED_KIND[node.op_name], e.nb + e.nbt))
             descs_t = (torch.frombuffer(bytearray(b"".join(descs)), dtype=torch.uint8).to(dev)
                        if descs else None)
-            args = struct.pack(f"<Qii{max(len(params), 1)}Q", units_t.data_ptr(), len(units), 0,
-                               *([p.data_ptr() for p in params] or [0]))
+            args = struct.pack(f"<3Q{max(len(params), 1)}Q", units_t.data_ptr(), qstart_t.data_ptr(),
+                               counters.data_ptr(), *([p.data_ptr() for p in params] or [0]))
             self.launches.append({"params": params, "units": units_t, "nunits": len(units),
+                                  "qstart": qstart_t, "counters": counters,
                                   "descs": descs_t, "ndescs": len(descs), "args": args,
                                   "ncu": ncu, "grid": None})
+
+    @staticmethod
+    def _cost_weight(gen: "F.KernelGen") -> float:
+        """Relative cost per voxel of a structure: transcendentals (FP64, per tap) dominate."""
+        from .tune import group_stats
+        st = group_stats(gen)
+        return 1.0 + 0.25 * st["taps"] + 8.0 * st["transc"]
 
     # -- execution -------------------------------------------------------------------------------
     def compile(self, workers: int = 1) -> None:

@@ -797,7 +797,7 @@ class _Emit:
         if op == "trace":
             return [self.var(f"sym3_trace({arr})")]
         if op == "determinant":
-            return [self.var(f"sym3_det_regularised({arr}, p.det_reg)")]
+            return [self.var(f"sym3_det_regularised({arr}, A.detreg)")]
         if op == "eigenvalues":
             e = f"e{self.n}"
             self.n += 1
@@ -979,8 +979,10 @@ class KernelGen:
         zlo = min(r, n0)
         zhi = max(n0 - r, zlo)
         if self.edge_mode == "band":
+            # partial tiles included: own and halo loads wrap, so cells past the grid edge
+            # hold the periodic images and interior formulas stay exact up to the band
             ylo, xlo = min(r, n1), min(r, n2)
-            yhi, xhi = max(min(yp, n1 - r), ylo), max(min(xp, n2 - r), xlo)
+            yhi, xhi = max(n1 - r, ylo), max(n2 - r, xlo)
         else:
             ylo, yhi, xlo, xhi = 0, yp, 0, xp
         return zlo, zhi, ylo, yhi, xlo, xhi
@@ -1062,9 +1064,12 @@ class KernelGen:
             boxes.append((zlo, zhi, ylo, yhi, xhi, n2))
         return [b for b in boxes if b[1] > b[0] and b[3] > b[2] and b[5] > b[4]]
 
+    TAIL_VPT = 4     # voxels per thread of a tail block
+
     def boundary_blocks(self, grid=None) -> int:
         tot = sum((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]) for b in self.boundary_boxes(grid))
-        return (tot + self.block_threads() - 1) // self.block_threads()
+        per = self.block_threads() * self.TAIL_VPT
+        return (tot + per - 1) // per
 
     @property
     def split_tail(self) -> bool:
@@ -1075,7 +1080,8 @@ class KernelGen:
         g = self.group
         boxes = self.boundary_boxes()
         A("    if (b >= nb) {")
-        A("        const i64 t = (i64)(b - nb) * NTB + tid;")
+        A(f"      for (int it_ = 0; it_ < {self.TAIL_VPT}; ++it_) {{")
+        A(f"        const i64 t = ((i64)(b - nb) * {self.TAIL_VPT} + it_) * NTB + tid;")
         A("        int z = -1, y = 0, x = 0;")
         if self.batched:
             # boxes of this grid, from the parameters: bstart = prefix sums of box sizes
@@ -1099,20 +1105,24 @@ class KernelGen:
         qa = "".join(", nullptr, nullptr, nullptr" for _ in self.queue_sources())
         A("        if (z >= 0) {")
         A("            const IDX vox = ((IDX)z * N1 + y) * N2 + x;")
-        A(f"            ev_gather(p, A, 0, 0, z, y, x, 0, 0, 0, 0, vox{qa});")
+        A(f"            {self._variant_name(('gather',))}(p, A, 0, 0, z, y, x, 0, 0, 0, 0, vox{qa});")
         A("        }")
+        A("      }")
         self._write_records(A, ndet, "        ")
         A("        return;")
         A("    }")
 
     def _write_records(self, A, ndet: int, ind: str) -> None:
+        flag = getattr(self, "_flag", False)
+        if flag:
+            for i in range(ndet):
+                A(f"{ind}{{ RMinMax fm; fm.lo = INFINITY; fm.hi = A.dflag{i} > 0.0f ? 1.0f : -INFINITY;"
+                  f" block_mm(fm, red_scratch, p.dflag{i} + (i64)rb * 4); }}")
+            return
         for ri, nid in enumerate(self.group.reductions):
             op = self.graph.node(nid).op_name
             fn = {"variance": "block_var", "min_max": "block_mm"}.get(op, "block_sum")
             A(f"{ind}{fn}(A.r{ri}, red_scratch, p.red{ri} + (i64)rb * 4);")
-        for i in range(ndet):
-            A(f"{ind}{{ RMinMax fm; fm.lo = INFINITY; fm.hi = A.dflag{i} > 0.0f ? 1.0f : -INFINITY;"
-              f" block_mm(fm, red_scratch, p.dflag{i} + (i64)rb * 4); }}")
 
     def _walk(self, e: "_Emit") -> None:
         pos = e.origin()
@@ -1137,7 +1147,8 @@ class KernelGen:
         e = _Emit(self, mode)
         pos = e.origin()
         body = []
-        for oi, nid in enumerate(self.group.outputs):
+        flag = getattr(self, "_flag", False)
+        for oi, nid in enumerate([] if flag else self.group.outputs):
             vals = e.val(nid, pos)
             C = len(vals)
             bf = self.out_dtypes.get(nid) == torch.bfloat16
@@ -1145,7 +1156,7 @@ class KernelGen:
                 st = "st_bf16" if bf else "st_f32"
                 cast = "(unsigned short*)" if bf else "(float*)"
                 body.append(f"{st}({cast}p.out{oi}, vox * {C} + {c}, {v});")
-        for ri, nid in enumerate(self.group.reductions):
+        for ri, nid in enumerate([] if flag else self.group.reductions):
             node = self.graph.node(nid)
             vals = e.val(node.input_ids[0], pos)
             if node.op_name == "integrate":
@@ -1160,7 +1171,7 @@ class KernelGen:
             else:
                 for v in vals:
                     body.append(f"A.r{ri}.add({v});")
-        for ki, nid in enumerate(self._det_inputs()):
+        for ki, nid in enumerate(self._det_inputs() if flag else []):
             vals = e.val(nid, pos)
             body.append(f"if (fabsf({vals[0]}) < 1e-12f || fabsf({vals[4]}) < 1e-12f || "
                         f"fabsf({vals[8]}) < 1e-12f) A.dflag{ki} = 1.0f;")
@@ -1174,13 +1185,13 @@ class KernelGen:
               f"    {sc}\n        {code}\n}}\n")
         return name, fn
 
-    @staticmethod
-    def _variant_name(mode) -> str:
+    def _variant_name(self, mode) -> str:
+        sfx = "_flag" if getattr(self, "_flag", False) else ""
         if mode[0] == "gather":
-            return "ev_gather"
+            return "ev_gather" + sfx
         if mode[0] == "xface":
-            return f"ev_x{mode[1]}{mode[2]}"
-        return f"ev_lds{int(mode[1])}{int(mode[2])}"
+            return f"ev_x{mode[1]}{mode[2]}" + sfx
+        return f"ev_lds{int(mode[1])}{int(mode[2])}" + sfx
 
     def main_boxes(self, grid=None) -> List[Tuple[int, int, int, int, int, int]]:
         """Boxes of voxels the main kernel evaluates (within full tiles)."""
@@ -1263,7 +1274,9 @@ class KernelGen:
         pf += [f"    double* red{i};" for i in range(nr)]
         pf += [f"    double* dflag{i};" for i in range(ndet)]
         pf += [f"    const double* scal{i};" for i in range(nd)]
-        pf.append("    int tiles_x, tiles_y, zchunks, nblocks;\n    float det_reg; int pad_;")
+        if ndet:
+            pf.append("    const double* detflag;      // 1.0: any |H_ii| < 1e-12 (from the flag pass)")
+        pf.append("    int tiles_x, tiles_y, zchunks, nblocks;\n    float unused_; int pad_;")
         if self.batched:
             pf.append("    int n0, n1, n2, ylo, yhi, xlo, xhi, nbt;\n    int bstart[8];\n    int boxes[36];")
         lines.append("struct Params {\n" + "\n".join(pf) + "\n};\n")
@@ -1274,6 +1287,7 @@ class KernelGen:
             acc.append(f"    {ty} r{ri};")
         acc += [f"    float dflag{i};" for i in range(ndet)]
         acc += [f"    float sc{i};" for i in range(nd)]
+        acc.append("    float detreg;")
         acc.append("    int dummy_;")
         lines.append("struct Acc {\n" + "\n".join(acc) + "\n};\n")
         off = 0
@@ -1295,6 +1309,21 @@ class KernelGen:
         lines.append(self._kernel(fns, ndet))
         if self.split_tail:
             lines.append(self._tail_kernel(ndet))
+        if ndet and not self.batched:
+            # flag pass: the same z-march evaluating only the determinant inputs' diagonals;
+            # its any-small-diagonal result sets the main pass's regularisation on the device
+            self._flag = True
+            try:
+                ffns = []
+                for m in self._variants():
+                    name, fn = self._emit_variant(m)
+                    ffns.append((m, name))
+                    lines.append(fn)
+                lines.append(self._kernel(ffns, ndet))
+                if self.split_tail:
+                    lines.append(self._tail_kernel(ndet))
+            finally:
+                self._flag = False
         return lines
 
     def _tail_kernel(self, ndet: int) -> str:
@@ -1303,7 +1332,8 @@ class KernelGen:
         g = self.group
         out: List[str] = []
         A = out.append
-        A('extern "C" __global__ void __launch_bounds__(NTB) fused_tail(const Params p) {')
+        kname = "fused_flag_tail" if getattr(self, "_flag", False) else "fused_tail"
+        A(f'extern "C" __global__ void __launch_bounds__(NTB) {kname}(const Params p) {{')
         A("    __shared__ float red_scratch[4 * NTB / 64 + 4];")
         A("    const int tid = threadIdx.x, nb = p.nblocks, b = nb + (int)blockIdx.x, rb = b;")
         A("    Acc A;")
@@ -1311,6 +1341,9 @@ class KernelGen:
             A(f"    A.r{ri}.init();")
         for i in range(ndet):
             A(f"    A.dflag{i} = 0.0f;")
+        # the global +1e-12 I regularisation of determinants, decided by the flag pass
+        A("    A.detreg = p.detflag[0] > 0.0 ? 1e-12f : 0.0f;" if ndet and not getattr(self, "_flag", False)
+          else "    A.detreg = 0.0f;")
         for i in range(len(g.dscalars)):
             A(f"    A.sc{i} = (float)p.scal{i}[0];")
         self._boundary_part(A, ndet)
@@ -1328,7 +1361,8 @@ class KernelGen:
             A("    const int tid = threadIdx.x, rb = b;")
             A("    const int nb = p.nblocks;")
         else:
-            A('extern "C" __global__ void __launch_bounds__(NTB) fused_kernel(const Params p) {')
+            kname = "fused_flag" if getattr(self, "_flag", False) else "fused_kernel"
+            A(f'extern "C" __global__ void __launch_bounds__(NTB) {kname}(const Params p) {{')
             A("    __shared__ float red_scratch[4 * NTB / 64 + 4];")
             A("    const int tid = threadIdx.x, rb = blockIdx.x;")
             # XCD-aware block remap: XCD k (hardware block id % 8) walks a contiguous tile range
@@ -1338,6 +1372,9 @@ class KernelGen:
             A(f"    A.r{ri}.init();")
         for i in range(ndet):
             A(f"    A.dflag{i} = 0.0f;")
+        # the global +1e-12 I regularisation of determinants, decided by the flag pass
+        A("    A.detreg = p.detflag[0] > 0.0 ? 1e-12f : 0.0f;" if ndet and not getattr(self, "_flag", False)
+          else "    A.detreg = 0.0f;")
         for i in range(len(g.dscalars)):
             A(f"    A.sc{i} = (float)p.scal{i}[0];")
         if self.use_lds and not self.split_tail:
@@ -1370,7 +1407,7 @@ class KernelGen:
         # partial tiles (last tile of an axis not divisible by the tile) use the gather
         # variant: one-sided edge formulas there reach inward past the tile start
         A("    const bool partial = (x0 + TX > N2) || (y0 + TY > N1);")
-        if self.use_lds:
+        if self.use_lds and self.edge_mode != "band":
             # partial tiles are evaluated by the boundary kernel; write empty records
             A("    if (partial) {")
             for ri, nid in enumerate(g.reductions):
@@ -1415,11 +1452,13 @@ class KernelGen:
             HT = self._halo_count(i)
             KH = (HT + s.threads - 1) // s.threads
             L_["KH"] = KH
-            # own cells: plane-relative offsets (clamped so partial tiles stay in bounds)
+            # own cells: plane-relative offsets; cells past the grid edge (partial tiles) load
+            # their periodic images, like the halo (band mode), or the edge value (variants)
             A(f"    int og{i}[VPT];")
             A("    #pragma unroll")
             A("    for (int k = 0; k < VPT; ++k)")
-            A(f"        og{i}[k] = (clampc(y0 + ly0 + k * RP, N1) * N2 + clampc(x0 + lx, N2)) * {C};")
+            fn = "wrapm" if self.edge_mode == "band" else "clampc"
+            A(f"        og{i}[k] = ({fn}(y0 + ly0 + k * RP, N1) * N2 + {fn}(x0 + lx, N2)) * {C};")
             if KH:
                 A(f"    int hg{i}[{KH}], hl{i}[{KH}];")
                 A("    #pragma unroll")
@@ -1512,16 +1551,17 @@ class KernelGen:
         if self.edge_mode == "band" and ("lds", False, False) in names:
             # z-edge planes, the y face band and corners belong to the tail blocks; the
             # x-face band to the face wave
-            A(f"{ind}      if (!ez && y >= YLO && y < YHI && x >= XLO && x < XHI) ev_lds00{call}")
+            A(f"{ind}      if (!ez && y >= YLO && y < YHI && x >= XLO && x < XHI) {self._variant_name(('lds', False, False))}{call}")
         elif ("lds", False, False) in names:
             # z-edge planes belong to the boundary kernel
             A(f"{ind}      if (ez) {{ }}")
-            A(f"{ind}      else if (!ex && !ey) ev_lds00{call}")
-            A(f"{ind}      else if (ex && !ey) ev_lds10{call}")
-            A(f"{ind}      else if (!ex) ev_lds01{call}")
-            A(f"{ind}      else ev_lds11{call}")
+            vn = lambda ex_, ey_: self._variant_name(("lds", ex_, ey_))
+            A(f"{ind}      else if (!ex && !ey) {vn(False, False)}{call}")
+            A(f"{ind}      else if (ex && !ey) {vn(True, False)}{call}")
+            A(f"{ind}      else if (!ex) {vn(False, True)}{call}")
+            A(f"{ind}      else {vn(True, True)}{call}")
         else:
-            A(f"{ind}      ev_gather{call}")
+            A(f"{ind}      {self._variant_name(('gather',))}{call}")
         A(f"{ind}    }}")
         A(f"{ind}  }}")
         if self.fw:
@@ -1535,7 +1575,7 @@ class KernelGen:
             for side in self.xface_sides():
                 for d in range(self.redge):
                     xc = d if side == "lo" else self.grid[2] - 1 - d
-                    A(f"{ind}      {'if' if first else 'else if'} (x == {xc}) ev_x{side}{d}{call}")
+                    A(f"{ind}      {'if' if first else 'else if'} (x == {xc}) {self._variant_name(('xface', side, d))}{call}")
                     first = False
             A(f"{ind}    }}")
             A(f"{ind}  }}")
@@ -1610,13 +1650,21 @@ class CompiledGroup:
     gen: KernelGen
     source: str
     module: Optional[L.Module] = None
-    tail: Optional[L.Module] = None
+    others: Dict[str, L.Module] = dc_field(default_factory=dict)
+
+    def kernel(self, name: str) -> L.Module:
+        """Another entry point of the same code object (fused_tail, fused_flag, ...)."""
+        if name == "fused_kernel":
+            if self.module is None:
+                self.module = load_module(self.source)
+            return self.module
+        if name not in self.others:
+            code, _ = compile_source(self.source)
+            self.others[name] = L.Module(code, name)
+        return self.others[name]
 
     def tail_module(self) -> L.Module:
-        if self.tail is None:
-            code, _ = compile_source(self.source)
-            self.tail = L.Module(code, "fused_tail")
-        return self.tail
+        return self.kernel("fused_tail")
 
     @property
     def group(self) -> Group:
@@ -1631,10 +1679,13 @@ class CompiledGroup:
 
 def pack_params(cg: CompiledGroup, srcs: Sequence[torch.Tensor], outs: Sequence[torch.Tensor],
                 reds: Sequence[torch.Tensor], dflags: Sequence[torch.Tensor],
-                scal: Sequence[torch.Tensor], det_reg: float) -> bytes:
+                scal: Sequence[torch.Tensor], det_reg: float,
+                detflag: Optional[torch.Tensor] = None) -> bytes:
     tx, ty, zcn, nb = cg.nblocks()
     ptrs = [t.data_ptr() for t in srcs] + [t.data_ptr() for t in outs] + \
            [t.data_ptr() for t in reds] + [t.data_ptr() for t in dflags] + [t.data_ptr() for t in scal]
+    if dflags:
+        ptrs.append(detflag.data_ptr())
     return struct.pack(f"<{len(ptrs)}Q4ifi", *ptrs, tx, ty, zcn, nb, float(det_reg), 0)
 
 
@@ -1755,6 +1806,7 @@ class FusedExecutor:
         reds = [torch.empty(nrec * 4, dtype=torch.float64, device=dev) for _ in g.reductions]
         ndet = len(cg.gen._det_inputs())
         dflags = [torch.empty(nrec * 4, dtype=torch.float64, device=dev) for _ in range(ndet)]
+        detflag = torch.zeros(1, dtype=torch.float64, device=dev) if ndet else None
         scal = []
         for d in g.dscalars:
             t = results[d]
@@ -1762,7 +1814,7 @@ class FusedExecutor:
                 t = torch.tensor(float(t))
             scal.append(t.to(device=dev, dtype=torch.float64).reshape(1).contiguous())
         side = self.side_stream() if cg.gen.split_tail else None
-        return Launch(cg, srcs, outs, reds, dflags, scal, nb, side)
+        return Launch(cg, srcs, outs, reds, dflags, scal, nb, side, detflag)
 
     def _run_group(self, cg: CompiledGroup, graph: Graph, results: Dict[str, Any]) -> None:
         g = cg.group
@@ -1770,14 +1822,13 @@ class FusedExecutor:
         dev = self.device
         ln = self.prepare_launch(cg, graph, results)
         stream = L.stream_ptr(dev)
-        ln.launch(stream)
         nb = ln.nblocks + cg.gen.boundary_blocks()
         if ln.dflags:
-            flag = torch.empty(2, dtype=torch.float64, device=dev)
+            # flag pass, then the device-side any-small-diagonal flag the main pass reads
+            ln.launch(stream, entry="fused_flag")
             L.check(L.lib().sp_reduce_combine(L.REDUCE["any_small_diag"], ln.dflags[0].data_ptr(), nb,
-                                              n0 * n1 * n2, flag.data_ptr(), stream), "det flag")
-            if float(flag[1].item()) > 0.0:
-                ln.launch(stream, det_reg=1e-12)
+                                              n0 * n1 * n2, ln.detflag.data_ptr(), stream), "det flag")
+        ln.launch(stream)
         for o, t in zip(g.outputs, ln.outs):
             results[o] = t
         for r, ws in zip(g.reductions, ln.reds):
@@ -1802,25 +1853,29 @@ class Launch:
     scal: List[torch.Tensor]
     nblocks: int
     side: Optional["torch.cuda.Stream"] = None
+    detflag: Optional[torch.Tensor] = None
 
-    def args(self, det_reg: float = 0.0) -> bytes:
-        return pack_params(self.cg, self.srcs, self.outs, self.reds, self.dflags, self.scal, det_reg)
+    def args(self) -> bytes:
+        return pack_params(self.cg, self.srcs, self.outs, self.reds, self.dflags, self.scal, 0.0,
+                           self.detflag)
 
-    def launch(self, stream: Optional[int] = None, det_reg: float = 0.0) -> None:
-        """Main kernel on ``stream`` (default: torch's current stream); the tail kernel, when
-        split, on the side stream between a fork and a join with ``stream``."""
+    def launch(self, stream: Optional[int] = None, entry: str = "fused_kernel") -> None:
+        """Main kernel (or the flag pass, ``entry="fused_flag"``) on ``stream`` (default:
+        torch's current stream); the tail kernel, when split, on the side stream between a
+        fork and a join with ``stream``."""
         gen = self.cg.gen
         dev = self.srcs[0].device if self.srcs else self.outs[0].device
         st = stream if stream is not None else L.stream_ptr(dev)
-        a = self.args(det_reg)
+        a = self.args()
         nbt = gen.boundary_blocks()
+        tail = "fused_flag_tail" if entry == "fused_flag" else "fused_tail"
         if self.side is None or not nbt:
-            self.cg.module.launch((self.nblocks + nbt, 1, 1), (gen.block_threads(), 1, 1), a, 0, st)
+            self.cg.kernel(entry).launch((self.nblocks + nbt, 1, 1), (gen.block_threads(), 1, 1), a, 0, st)
             return
         main = torch.cuda.current_stream(dev) if stream is None else torch.cuda.ExternalStream(st, device=dev)
         self.side.wait_stream(main)
-        self.cg.module.launch((self.nblocks, 1, 1), (gen.block_threads(), 1, 1), a, EXTRA_LDS, st)
-        self.cg.tail_module().launch((nbt, 1, 1), (gen.block_threads(), 1, 1), a, 0, self.side.cuda_stream)
+        self.cg.kernel(entry).launch((self.nblocks, 1, 1), (gen.block_threads(), 1, 1), a, EXTRA_LDS, st)
+        self.cg.kernel(tail).launch((nbt, 1, 1), (gen.block_threads(), 1, 1), a, 0, self.side.cuda_stream)
         main.wait_stream(self.side)
 
     def time_ms(self, reps: int = 20, warmup: int = 3, rounds: int = 3) -> float:

@@ -8,8 +8,10 @@ evaluated on each padded slab and cropped, min/max/mean of the sink reported.  D
 
 * the halo is the graph's *composite* stencil radius (the reference uses the largest single-op
   radius, wrong for chained stencils);
-* a producer thread fills ``n_buffers`` pinned host buffers (generator or file), the copy to
-  the device runs on a side stream and overlaps the fused compute of the previous slab;
+* a producer thread fills ``n_buffers`` pinned staging chunks (generator or file) that are
+  copied on a side stream into two device slab buffers, so the copy and generation of the
+  next slab overlap the fused compute of the current one and the pinned footprint stays a
+  few GiB even for 17 GB slabs;
 * a producer or writer exception is re-raised in the caller (the reference's reader thread
   can deadlock the pipeline, SURVEY Appendix B item 12);
 * sink reductions (sum of squares for the L2 norm, sum, min, max) are taken over the cropped
@@ -119,18 +121,32 @@ def halo_ranges(start: int, end: int, total: int, radius: int, boundary: str):
 
 
 class _Slot:
+    """One pinned host staging buffer."""
+
     def __init__(self, planes: int, plane_shape: Tuple[int, ...]):
         self.host = torch.empty((planes,) + plane_shape, dtype=torch.float32, pin_memory=True)
         self.free = threading.Event()
         self.free.set()
-        self.copied: Optional[torch.cuda.Event] = None
+
+
+def _release_after(ev: "torch.cuda.Event", slot: _Slot) -> None:
+    ev.synchronize()
+    slot.free.set()
 
 
 def stream_graph(graph: Graph, sink: str, src_id: str, lazy, out_path: Optional[str] = None,
                  rt: Optional[Runtime] = None, boundary: str = "periodic",
                  chunk_mb: Optional[float] = None, slab_planes: Optional[int] = None,
-                 n_buffers: int = 3, max_slabs: Optional[int] = None) -> Dict[str, Any]:
-    """Evaluate ``graph`` over ``lazy`` (the source node ``src_id``) slab by slab."""
+                 n_buffers: int = 3, max_slabs: Optional[int] = None,
+                 stage_mb: float = 1024.0) -> Dict[str, Any]:
+    """Evaluate ``graph`` over ``lazy`` (the source node ``src_id``) slab by slab.
+
+    A slab (``slab_planes`` planes plus the composite-radius halo) is assembled on the device
+    from pinned staging chunks of at most ``stage_mb`` MiB: a producer thread fills
+    ``n_buffers`` pinned chunks from the generator or file, each chunk is copied on a side
+    stream into one of two device slab buffers, and the fused graph runs on a slab as soon
+    as its last chunk has landed -- while the next slab's chunks are being produced and
+    copied into the other buffer."""
     rt = rt or Runtime(device="cuda")
     dev = rt.device
     n0 = lazy.shape[0]
@@ -142,11 +158,13 @@ def stream_graph(graph: Graph, sink: str, src_id: str, lazy, out_path: Optional[
         slab_planes = max(2 * r + 1, min(n0, budget // plane_bytes))
     slab_planes = min(max(1, slab_planes), n0)
     cap = slab_planes + 2 * r
-    slabs = [(s, min(s + slab_planes, n0)) for s in range(0, n0, slab_planes)]
+    stage = max(1, min(cap, int(stage_mb * 2 ** 20) // plane_bytes))
+    slabs = [(s_, min(s_ + slab_planes, n0)) for s_ in range(0, n0, slab_planes)]
     if max_slabs is not None:
         slabs = slabs[:max_slabs]
-    slots = [_Slot(cap, plane_shape) for _ in range(max(2, n_buffers))]
+    slots = [_Slot(stage, plane_shape) for _ in range(max(2, n_buffers))]
     dev_bufs = [torch.empty((cap,) + plane_shape, dtype=torch.float32, device=dev) for _ in range(2)]
+    buf_done = [None, None]              # compute-finished event of each device slab buffer
     copy_stream = torch.cuda.Stream(device=dev)
     ready: "queue.Queue" = queue.Queue(maxsize=len(slots))
     stop = threading.Event()
@@ -154,64 +172,81 @@ def stream_graph(graph: Graph, sink: str, src_id: str, lazy, out_path: Optional[
 
     def producer():
         try:
-            for k, (s, e) in enumerate(slabs):
-                slot = slots[k % len(slots)]
-                while not slot.free.wait(0.1):
-                    if stop.is_set():
-                        return
-                slot.free.clear()
-                pieces, lp, rp = halo_ranges(s, e, n0, r, boundary)
-                t0 = time.perf_counter()
+            n = 0
+            for k, (s_, e_) in enumerate(slabs):
+                pieces, lp, rp = halo_ranges(s_, e_, n0, r, boundary)
+                runs = [(a, b) for a, b in pieces]
+                planes = sum(b - a for a, b in runs)
                 off = 0
-                buf = slot.host.numpy()
-                for a, b in pieces:
-                    lazy.read_into(a, b, buf[off:off + (b - a)])
-                    off += b - a
-                t_gen[0] += time.perf_counter() - t0
-                ready.put((k, s, e, off, lp, rp, None))
+                for a, b in runs:
+                    z = a
+                    while z < b:
+                        cnt = min(stage, b - z)
+                        slot = slots[n % len(slots)]
+                        n += 1
+                        while not slot.free.wait(0.1):
+                            if stop.is_set():
+                                return
+                        slot.free.clear()
+                        t0 = time.perf_counter()
+                        lazy.read_into(z, z + cnt, slot.host[:cnt].numpy())
+                        t_gen[0] += time.perf_counter() - t0
+                        last = off + cnt == planes
+                        ready.put(("chunk", k, slot, off, cnt, (s_, e_, planes, lp, rp) if last else None))
+                        off += cnt
+                        z += cnt
         except BaseException as exc:  # propagate to the consumer instead of hanging it
-            ready.put((None, 0, 0, 0, 0, 0, exc))
+            ready.put(("error", exc))
 
     th = threading.Thread(target=producer, daemon=True)
     th.start()
-    writer = None
     if out_path is not None:
         out_mm = np.lib.format.open_memmap(out_path, mode="w+", dtype=np.float32,
                                            shape=(n0,) + plane_shape)
     sumsq = sums = 0.0
     mn, mx = math.inf, -math.inf
     count = 0
+    done_slabs = 0
     t_start = time.perf_counter()
+    compute = torch.cuda.current_stream(dev)
     try:
-        for _ in slabs:
-            k, s, e, planes, lp, rp, exc = ready.get()
-            if exc is not None:
-                raise RuntimeError(f"slab producer failed: {exc}") from exc
-            slot = slots[k % len(slots)]
-            dbuf = dev_bufs[k % 2][:planes]
+        while done_slabs < len(slabs):
+            msg = ready.get()
+            if msg[0] == "error":
+                raise RuntimeError(f"slab producer failed: {msg[1]}") from msg[1]
+            _, k, slot, off, cnt, fin = msg
+            dbuf = dev_bufs[k % 2]
             with torch.cuda.stream(copy_stream):
-                dbuf.copy_(slot.host[:planes], non_blocking=True)
+                if off == 0 and buf_done[k % 2] is not None:
+                    copy_stream.wait_event(buf_done[k % 2])     # slab k-2 finished with it
+                dbuf[off:off + cnt].copy_(slot.host[:cnt], non_blocking=True)
                 ev = torch.cuda.Event()
                 ev.record(copy_stream)
-            torch.cuda.current_stream(dev).wait_event(ev)
-            # the host slot can be refilled once its copy has finished
-            threading.Thread(target=lambda ev=ev, slot=slot: (ev.synchronize(), slot.free.set()),
-                             daemon=True).start()
-            g = graph.clone_with_replacement(src_id, dbuf)
+            # the pinned chunk can be refilled once its copy has finished
+            threading.Thread(target=_release_after, args=(ev, slot), daemon=True).start()
+            if fin is None:
+                continue
+            s_, e_, planes, lp, rp = fin
+            compute.wait_event(ev)
+            g = graph.clone_with_replacement(src_id, dbuf[:planes])
             res = rt.run(g, outputs=[sink])
-            out = res[sink][lp:lp + (e - s)]
+            out = res[sink][lp:lp + (e_ - s_)]
             if not out.is_contiguous():
                 out = out.contiguous()
             n2 = L.reduce("norm_l2", out)
             sm = L.reduce("sum", out)
             mm = L.reduce("min_max", out)
+            done = torch.cuda.Event()
+            done.record(compute)
+            buf_done[k % 2] = done
             sumsq += float(n2.item()) ** 2
             sums += float(sm.item())
             lo, hi = mm.tolist()
             mn, mx = min(mn, lo), max(mx, hi)
             count += out.numel()
             if out_path is not None:
-                out_mm[s:e] = out.cpu().numpy()
+                out_mm[s_:e_] = out.cpu().numpy()
+            done_slabs += 1
     finally:
         stop.set()
         th.join(timeout=60)
@@ -219,10 +254,10 @@ def stream_graph(graph: Graph, sink: str, src_id: str, lazy, out_path: Optional[
             out_mm.flush()
     torch.cuda.synchronize(dev)
     wall = time.perf_counter() - t_start
-    voxels = sum(e - s for s, e in slabs) * int(np.prod(plane_shape))
-    return {"shape_out": [sum(e - s for s, e in slabs)] + list(plane_shape), "n_chunks": len(slabs),
-            "halo": r, "slab_planes": slab_planes, "l2": math.sqrt(sumsq), "sum": sums,
-            "mean": sums / count if count else 0.0, "min": mn, "max": mx,
+    voxels = sum(e_ - s_ for s_, e_ in slabs) * int(np.prod(plane_shape))
+    return {"shape_out": [sum(e_ - s_ for s_, e_ in slabs)] + list(plane_shape), "n_chunks": len(slabs),
+            "halo": r, "slab_planes": slab_planes, "stage_planes": stage, "l2": math.sqrt(sumsq),
+            "sum": sums, "mean": sums / count if count else 0.0, "min": mn, "max": mx,
             "voxels": voxels, "wall_s": wall, "gvoxel_s": voxels / wall / 1e9,
             "host_gen_s": t_gen[0], "out_path": out_path}
 

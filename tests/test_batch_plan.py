@@ -24,13 +24,15 @@ def _graphs(n_items=8, n_graphs=4, seed=3, lo=32, hi=72):
     return exprs, out
 
 
-def test_xcd_order_is_a_permutation_with_contiguous_ranges_per_xcd():
-    for n in (1, 7, 8, 100, 2049):
-        o = B.xcd_order(n)
-        assert sorted(o.tolist()) == list(range(n))
-        for x in range(8):
-            seq = o[x::8]
-            assert np.all(np.diff(seq) == 1)
+def test_queue_split_balances_cost_in_contiguous_runs():
+    rng = np.random.default_rng(0)
+    for n in (0, 1, 7, 100, 2049):
+        c = rng.uniform(1, 10, n)
+        b = B.queue_split(c, 8)
+        assert b[0] == 0 and b[-1] == n and len(b) == 9 and np.all(np.diff(b) >= 0)
+        if n >= 100:
+            sums = [c[b[q]:b[q + 1]].sum() for q in range(8)]
+            assert max(sums) - min(sums) <= 2 * c.max()
 
 
 def test_batch_plan_units_cover_every_tile_and_tail_block():
@@ -44,6 +46,9 @@ def test_batch_plan_units_cover_every_tile_and_tail_block():
             continue
         u = ln["units"].numpy()
         assert u.shape == (ln["nunits"], 4)
+        qs = ln["qstart"].numpy()
+        assert qs[0] == 0 and qs[-1] == ln["nunits"] and np.all(np.diff(qs) >= 0)
+        assert ln["counters"].numpy().tolist() == [0] * 9
         keys = br.mods[lvl]["keys"]
         for ki, k in enumerate(keys):
             es = [e for e in entries if e.key == k]

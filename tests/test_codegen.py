@@ -87,8 +87,8 @@ def test_main_and_boundary_kernels_cover_every_voxel_once(shape, sched, edge_mod
     for (c0, c1, b0, b1, a0, a1) in gen.main_boxes():
         for ty in range(0, n1, sched.ty):
             for tx in range(0, n2, sched.tx):
-                if ty + sched.ty > n1 or tx + sched.tx > n2:
-                    continue
+                if (ty + sched.ty > n1 or tx + sched.tx > n2) and edge_mode != "band":
+                    continue        # partial tiles: tail blocks (band mode: the main kernel)
                 ya, yb = max(ty, b0), min(ty + sched.ty, b1)
                 xa, xb = max(tx, a0), min(tx + sched.tx, a1)
                 if ya < yb and xa < xb and c0 < c1:
@@ -100,5 +100,5 @@ def test_main_and_boundary_kernels_cover_every_voxel_once(shape, sched, edge_mod
     r = gen.redge
     assert z0 >= r and z1 <= n0 - r or z0 >= z1
     assert y0 >= min(r, n1) * (edge_mode == "band") and x0 >= min(r, n2) * (edge_mode == "band")
-    assert gen.boundary_blocks() * gen.block_threads() >= sum((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4])
+    assert gen.boundary_blocks() * gen.block_threads() * gen.TAIL_VPT >= sum((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4])
                                               for b in gen.boundary_boxes())

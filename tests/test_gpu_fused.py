@@ -103,3 +103,31 @@ def test_fused_matches_eager_path():
     ea, _ = errs(a, o64, o64.float())
     eb, _ = errs(b, o64, o64.float())
     assert ea <= 2 * eb + 1e-12, (ea, eb)
+
+
+def test_determinant_regularises_only_when_a_diagonal_is_small(monkeypatch):
+    """The fused determinant adds 1e-12 I only when the any-small-diagonal flag is set
+    (stenpy_engine.py:2683-2712): the flag pass runs first and the main pass reads its device
+    flag -- 0 for a generic field, 1 for f = x^2 (H_yy = H_zz = 0)."""
+    from stenpy_amd import fused as F
+    launches = []
+    orig = F.FusedExecutor.prepare_launch
+
+    def keep(self, cg, graph, results):
+        ln = orig(self, cg, graph, results)
+        launches.append(ln)
+        return ln
+
+    monkeypatch.setattr(F.FusedExecutor, "prepare_launch", keep)
+    shape = (20, 24, 30)
+    run_and_check("determinant(hessian(f))", {"f": field(shape, 8)}, boundary="periodic")
+    assert float(launches[-1].detflag.item()) == 0.0
+    x = torch.arange(shape[0], dtype=torch.float32).view(-1, 1, 1).expand(shape).contiguous()
+    f = (x * x).cuda()
+    g, sink, _, _, _ = sp.compile_expression("determinant(hessian(f))", boundary="periodic", field_map={"f": f})
+    rt = sp.Runtime(device="cuda", fusion_enabled=True)
+    out = rt.fused.run(g, rt, wanted=[sink])[sink]
+    assert float(launches[-1].detflag.item()) == 1.0
+    # H = diag(2, 0, 0) exactly, so det(H + 1e-12 I) = (2 + 1e-12) * 1e-24
+    ref = graph_eval.evaluate(g, torch.float64)[sink]
+    assert (out.double().cpu() - ref).abs().max().item() <= 1e-30

@@ -62,8 +62,9 @@ C4 = "f**2 + tanh(f)*exp(f) + laplacian(f)"
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("stage_mb", [1024.0, 0.03])       # whole slabs / 3-plane staging chunks
 @pytest.mark.parametrize("slab", [7, 16, 40])
-def test_streamed_equals_whole_field(slab, tmp_path):
+def test_streamed_equals_whole_field(slab, stage_mb, tmp_path):
     shape = (40, 36, 70)
     lf = GeneratorField(shape, 7)
     whole = torch.from_numpy(lf[0:40]).cuda()
@@ -72,12 +73,13 @@ def test_streamed_equals_whole_field(slab, tmp_path):
     probe = torch.zeros((1,) + shape[1:])
     g2, sink2, _, fm2, _ = sp.compile_expression(C4, boundary="periodic", field_map={"f": probe})
     outp = str(tmp_path / "out.npy")
-    st = stream_graph(g2, sink2, fm2["f"], lf, out_path=outp, slab_planes=slab)
+    st = stream_graph(g2, sink2, fm2["f"], lf, out_path=outp, slab_planes=slab, stage_mb=stage_mb)
     got = torch.from_numpy(np.load(outp))
     assert torch.equal(got, ref), (got - ref).abs().max()
     l2 = float(torch.linalg.vector_norm(ref.double()))
     assert abs(st["l2"] - l2) <= 1e-6 * l2
     assert st["n_chunks"] == -(-40 // slab) and st["halo"] == 1
+    assert st["stage_planes"] == (3 if stage_mb < 1 else slab + 2)
 
 
 @pytest.mark.gpu
