@@ -145,6 +145,10 @@ typedef struct sp_combine_desc {
     int32_t nrecords;
 } sp_combine_desc;
 int sp_reduce_combine_batch(const sp_combine_desc* descs, int n, void* stream);
+/* The same for at most SP_COMBINE_MULTI_MAX descriptors given in host memory (copied into the
+ * launch's arguments): one launch for all reductions of one fused group. */
+#define SP_COMBINE_MULTI_MAX 16
+int sp_reduce_combine_multi(const sp_combine_desc* descs, int n, void* stream);
 
 /* ---- per-voxel symmetric 3x3 tensor ops (stenpy_engine.py:2674-2730, 2852-2867) -------
  *   SP_T_TRACE -> (...), SP_T_DET -> (...) [``flag`` = device double from SP_R_ANYSMALLDIAG:

@@ -87,6 +87,7 @@ def _declare(lib: C.CDLL) -> None:
         "sp_reduce": ([I, P, V, V, C.POINTER(D), V], I),
         "sp_reduce_combine": ([I, V, I, C.c_int64, V, V], I),
         "sp_reduce_combine_batch": ([V, I, V], I),
+        "sp_reduce_combine_multi": ([V, I, V], I),
         "sp_tensor3": ([I, P, P, V, V], I),
         "sp_gen_fourier": ([P, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int32),
                             C.POINTER(D), C.POINTER(D), C.c_uint64, V], I),

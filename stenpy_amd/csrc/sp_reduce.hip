@@ -190,22 +190,41 @@ __device__ __forceinline__ void merge64(Acc64& a, const Acc64& b, bool minmax) {
     a.c2 += b.c2;
 }
 
+constexpr int kCThreads = 1024;   // threads of a final-combine block
+
+__device__ __forceinline__ Acc64 load_record(int kind, bool mm, const double* __restrict__ rec) {
+    Acc64 r{0, 0, 0, 0, INFINITY, -INFINITY};
+    if (kind == SP_R_VAR || kind == SP_R_VAR_BIASED) { r.s = rec[0]; r.c = rec[1]; r.s2 = rec[2]; r.c2 = rec[3]; }
+    else if (mm) { r.lo = rec[0]; r.hi = rec[1]; }
+    else { r.s = rec[0]; r.c = rec[1]; }
+    return r;
+}
+
+// Final combine of ``nblocks`` block records by one block of kCThreads threads.  Thread t
+// merges records t, t + kCThreads, ... into four independent accumulators (their loads are in
+// flight together), then the block merges the partials in a fixed tree: the result does not
+// depend on timing.
 __device__ __forceinline__ void combine_records(int kind, const double* __restrict__ ws, int nblocks,
                                                 int64_t n, double* __restrict__ result) {
     const bool mm = (kind == SP_R_MINMAX || kind == SP_R_ANYSMALLDIAG);
-    Acc64 a{0, 0, 0, 0, INFINITY, -INFINITY};
-    for (int b = threadIdx.x; b < nblocks; b += kThreads) {
-        const double* rec = ws + (int64_t)b * kRec;
-        Acc64 r{0, 0, 0, 0, INFINITY, -INFINITY};
-        if (kind == SP_R_VAR || kind == SP_R_VAR_BIASED) { r.s = rec[0]; r.c = rec[1]; r.s2 = rec[2]; r.c2 = rec[3]; }
-        else if (mm) { r.lo = rec[0]; r.hi = rec[1]; }
-        else { r.s = rec[0]; r.c = rec[1]; }
-        merge64(a, r, mm);
+    Acc64 a[4];
+    for (int k = 0; k < 4; ++k) a[k] = Acc64{0, 0, 0, 0, INFINITY, -INFINITY};
+    int b = threadIdx.x;
+    for (; b + 3 * kCThreads < nblocks; b += 4 * kCThreads) {
+        Acc64 r[4];
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = load_record(kind, mm, ws + (int64_t)(b + k * kCThreads) * kRec);
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) merge64(a[k], r[k], mm);
     }
-    __shared__ Acc64 part[kThreads];
-    part[threadIdx.x] = a;
+    for (; b < nblocks; b += kCThreads) merge64(a[0], load_record(kind, mm, ws + (int64_t)b * kRec), mm);
+    merge64(a[0], a[1], mm);
+    merge64(a[2], a[3], mm);
+    merge64(a[0], a[2], mm);
+    __shared__ Acc64 part[kCThreads];
+    part[threadIdx.x] = a[0];
     __syncthreads();
-    for (int step = kThreads / 2; step >= 1; step >>= 1) {
+    for (int step = kCThreads / 2; step >= 1; step >>= 1) {
         if (threadIdx.x < step) merge64(part[threadIdx.x], part[threadIdx.x + step], mm);
         __syncthreads();
     }
@@ -228,15 +247,22 @@ __device__ __forceinline__ void combine_records(int kind, const double* __restri
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_reduce_stage2(int kind, const double* __restrict__ ws,
+__global__ void __launch_bounds__(kCThreads) k_reduce_stage2(int kind, const double* __restrict__ ws,
                                                             int nblocks, int64_t n,
                                                             double* __restrict__ result) {
     combine_records(kind, ws, nblocks, n, result);
 }
 
 // one block per descriptor: the final combines of many reductions in one launch
-__global__ void __launch_bounds__(kThreads) k_reduce_combine_batch(const sp_combine_desc* __restrict__ d) {
+__global__ void __launch_bounds__(kCThreads) k_reduce_combine_batch(const sp_combine_desc* __restrict__ d) {
     const sp_combine_desc e = d[blockIdx.x];
+    combine_records(e.kind, e.records, e.nrecords, e.count, e.result);
+}
+
+// up to SP_COMBINE_MULTI_MAX descriptors passed by value (no device buffer; capturable)
+struct CombineArgs { sp_combine_desc d[SP_COMBINE_MULTI_MAX]; };
+__global__ void __launch_bounds__(kCThreads) k_reduce_combine_multi(const CombineArgs a) {
+    const sp_combine_desc e = a.d[blockIdx.x];
     combine_records(e.kind, e.records, e.nrecords, e.count, e.result);
 }
 
@@ -274,7 +300,7 @@ extern "C" int sp_reduce(int kind, const sp_tensor* x, double* result, void* wor
                        (const float*)x->ptr, n, (double*)workspace, ig);
     int rc = launch_status("sp_reduce(stage1)");
     if (rc) return rc;
-    hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(kThreads), 0, st, kind,
+    hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(kCThreads), 0, st, kind,
                        (const double*)workspace, (int)blocks, n, result);
     return launch_status("sp_reduce(stage2)");
 }
@@ -283,14 +309,23 @@ extern "C" int sp_reduce_combine(int kind, const double* records, int nrecords, 
                                  double* result, void* stream) {
     SP_REQUIRE(records && result && nrecords > 0, "reduce_combine: bad arguments");
     SP_REQUIRE(kind >= SP_R_SUM && kind <= SP_R_ANYSMALLDIAG, "reduce_combine: unknown kind");
-    hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(kThreads), 0, as_stream(stream), kind,
+    hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(kCThreads), 0, as_stream(stream), kind,
                        records, nrecords, count, result);
     return launch_status("sp_reduce_combine");
 }
 
 extern "C" int sp_reduce_combine_batch(const sp_combine_desc* descs, int n, void* stream) {
     SP_REQUIRE(descs && n > 0, "reduce_combine_batch: bad arguments");
-    hipLaunchKernelGGL(k_reduce_combine_batch, dim3((unsigned)n), dim3(kThreads), 0,
+    hipLaunchKernelGGL(k_reduce_combine_batch, dim3((unsigned)n), dim3(kCThreads), 0,
                        as_stream(stream), descs);
     return launch_status("sp_reduce_combine_batch");
+}
+
+extern "C" int sp_reduce_combine_multi(const sp_combine_desc* descs, int n, void* stream) {
+    SP_REQUIRE(descs && n > 0 && n <= SP_COMBINE_MULTI_MAX, "reduce_combine_multi: bad arguments");
+    CombineArgs a;
+    for (int i = 0; i < n; ++i) a.d[i] = descs[i];
+    hipLaunchKernelGGL(k_reduce_combine_multi, dim3((unsigned)n), dim3(kCThreads), 0,
+                       as_stream(stream), a);
+    return launch_status("sp_reduce_combine_multi");
 }

@@ -31,6 +31,7 @@ Kernel shape (one block = TY x TX voxels of a plane, marching along axis 0 over 
 """
 from __future__ import annotations
 
+import ctypes
 import hashlib
 import math
 import os
@@ -112,6 +113,7 @@ EDGE_MODE = os.environ.get("STENPY_EDGE_MODE", "auto")
 # z-marching kernel; off: the same work as extra blocks at the end of the main grid.
 SPLIT_TAIL = os.environ.get("STENPY_SPLIT_TAIL", "1") == "1"
 EXTRA_LDS = int(os.environ.get("STENPY_EXTRA_LDS", "0"))
+COMBINE_MULTI_MAX = 16        # descriptors per sp_reduce_combine_multi launch
 BAND_MAX_FRACTION = 0.08
 
 
@@ -1779,6 +1781,26 @@ class FusedExecutor:
             self._run_group(compiled[si], graph, results)
         return results
 
+    def capture(self, graph: Graph, runtime, wanted: Sequence[str],
+                out_dtype: Optional[torch.dtype] = None, warmup: int = 1):
+        """Capture one evaluation of ``graph`` as a hipGraph.
+
+        Returns (graph, results): replaying the graph re-evaluates into the same ``results``
+        tensors.  The plans are prepared and compiled by the warm-up runs; a fused run makes no
+        host synchronisation (reduction combines and the determinant flag stay on the device),
+        so every launch, including the tail kernels' fork/join on the side stream, is recorded.
+        Graphs containing non-fusable ops are captured only if those ops do not synchronise."""
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            for _ in range(warmup):
+                self.run(graph, runtime, wanted=wanted, out_dtype=out_dtype)
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            results = self.run(graph, runtime, wanted=wanted, out_dtype=out_dtype)
+        return g, results
+
     def prepare_launch(self, cg: CompiledGroup, graph: Graph, results: Dict[str, Any]) -> "Launch":
         """Buffers and packed arguments of one launch of ``cg`` (inputs read from ``results``)."""
         g = cg.group
@@ -1831,15 +1853,20 @@ class FusedExecutor:
         ln.launch(stream)
         for o, t in zip(g.outputs, ln.outs):
             results[o] = t
+        # every FP64 final combine of the group in one launch (descriptors passed by value)
+        descs = []
         for r, ws in zip(g.reductions, ln.reds):
             node = graph.node(r)
             inp = node.input_ids[0]
             count = n0 * n1 * n2 * max(cg.gen.info[inp].ch, 1)
             res = torch.empty(2 if node.op_name == "min_max" else 1, dtype=torch.float64, device=dev)
-            kind = RED_KIND[node.op_name]
-            L.check(L.lib().sp_reduce_combine(kind, ws.data_ptr(), nb, count, res.data_ptr(), stream),
-                    f"combine {node.op_name}")
+            descs.append(struct.pack("<QQqii", ws.data_ptr(), res.data_ptr(), count,
+                                     RED_KIND[node.op_name], nb))
             results[r] = res if node.op_name == "min_max" else res.reshape(())
+        for i in range(0, len(descs), COMBINE_MULTI_MAX):
+            chunk = descs[i:i + COMBINE_MULTI_MAX]
+            buf = ctypes.create_string_buffer(b"".join(chunk))
+            L.check(L.lib().sp_reduce_combine_multi(ctypes.addressof(buf), len(chunk), stream), "combine")
 
 
 @dataclass
