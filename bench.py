@@ -388,7 +388,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel_ms": round(k_ms, 4),
-                     "kernels": "fused_kernel (LDS z-march over full tiles + tail blocks for the face band)",
+                     "kernels": "fused_kernel (LDS z-march, face wave) + fused_tail (face band, side stream)",
                      "algorithmic_bytes_per_launch": alg_bytes},
     }
     if not args.no_cpu:
