@@ -112,7 +112,6 @@ EDGE_MODE = os.environ.get("STENPY_EDGE_MODE", "auto")
 # separate small kernel (no LDS arrays, few registers) on a side stream, beside the LDS
 # z-marching kernel; off: the same work as extra blocks at the end of the main grid.
 SPLIT_TAIL = os.environ.get("STENPY_SPLIT_TAIL", "1") == "1"
-EXTRA_LDS = int(os.environ.get("STENPY_EXTRA_LDS", "0"))
 COMBINE_MULTI_MAX = 16        # descriptors per sp_reduce_combine_multi launch
 BAND_MAX_FRACTION = 0.08
 
@@ -569,8 +568,12 @@ class _Emit:
         g = self.g
         if isinstance(nid, tuple):      # virtual nodes
             return self._virtual(nid, pos)
+        if nid in g.xform_node and self.mode[0] != "gather":
+            return [self.tap(g.xform_node[nid], 0, pos)]       # LDS holds the transformed value
         if nid in g.src_index:
             si = g.src_index[nid]
+            if si in g.xform and self.mode[0] != "gather":
+                raise RuntimeError("raw tap of a load-transformed source")
             return [self.tap(si, c, pos) for c in range(g.src_layout[si]["C"])]
         inf = g.info[nid]
         if inf.kind == "const":
@@ -951,6 +954,8 @@ class KernelGen:
         self.v = sched.ty // self.rp                 # voxels per thread per plane
         self.src_layout = [{"C": info[s].ch, "H": reach[s][2], "D": 2 * self.hz + 2,
                             "mode": "ring", "lds": True} for s in group.sources]
+        self.xform = self._load_transforms()
+        self.xform_node = {v[1]: si for si, v in self.xform.items()}
         # batched: one code for many grids (shapes are run-time values, see section())
         self.batched = batched
         if batched:
@@ -961,6 +966,80 @@ class KernelGen:
             edge_mode = "band"
         self.edge_mode = self._pick_edge_mode(edge_mode or EDGE_MODE)
         self._choose_modes()
+
+    _COSTLY = {"exp", "log", "sqrt", "sin", "cos", "tanh"}
+
+    def _load_transforms(self) -> Dict[int, Tuple[List[str], str]]:
+        """Scalar sources used only through a point-wise chain with a transcendental that feeds
+        a stencil (e.g. f in laplacian(tanh(f))): {source index: (chain, last node)}.
+
+        The chain is applied once per loaded cell (own and halo) before the LDS store, so the
+        stencil's taps read the transformed value instead of re-evaluating the chain -- in
+        FP64 for transcendentals -- at every tap.  Chain steps: unary ops, scaling, and binary
+        ops with a constant operand; each intermediate node has the next step as its only
+        consumer and is not an output."""
+        g, graph = self.group, self.graph
+        members = set(g.nodes)
+        cons: Dict[str, List[str]] = {}
+        for nid in g.nodes:
+            for i in graph.node(nid).input_ids:
+                if nid not in cons.setdefault(i, []):
+                    cons[i].append(nid)
+        ends = set(g.outputs) | {graph.node(r).input_ids[0] for r in g.reductions} | set(self._det_inputs())
+        out: Dict[int, Tuple[List[str], str]] = {}
+        for si, sid in enumerate(g.sources):
+            if self.info[sid].ch != 1:
+                continue
+            chain, cur = [], sid
+            while True:
+                cs = cons.get(cur, [])
+                if len(cs) != 1 or (cur != sid and cur in ends):
+                    break
+                node = graph.node(cs[0])
+                op, ins = node.op_name, node.input_ids
+                if op in UNARY_OPS and op != "cast_bf16":
+                    ok = True
+                elif op == "mul" and len(ins) == 1:
+                    ok = True
+                elif op in ("add", "sub", "mul", "div") and len(ins) == 2:
+                    other = ins[1] if ins[0] == cur else ins[0]
+                    ok = ins.count(cur) == 2 or self.info[other].kind == "const"
+                else:
+                    ok = False
+                if not ok or cs[0] not in members:
+                    break
+                chain.append(cs[0])
+                cur = cs[0]
+            if not chain or not any(graph.node(n).op_name in self._COSTLY for n in chain):
+                continue
+            last = chain[-1]
+            if not any(graph.node(c).op_name in STENCIL_OPS for c in cons.get(last, [])):
+                continue
+            out[si] = (chain, last)
+        return out
+
+    def _xform_functions(self) -> List[str]:
+        """``xf<i>(x)``: the load transform of source i (see _load_transforms)."""
+        fns = []
+        for si, (chain, last) in self.xform.items():
+            lines, cur, prev = [], "x", self.group.sources[si]
+            for k, nid in enumerate(chain):
+                node = self.graph.node(nid)
+                op, p_, ins = node.op_name, node.params, node.input_ids
+                if op in UNARY_OPS:
+                    e = _Emit._unary(op, cur, p_)
+                elif len(ins) == 1:
+                    e = f"{cur} * {_flt(p_['scalar'])}"
+                elif ins[0] == ins[1]:                       # x op x (e.g. f*f)
+                    e = _Emit._binary(op, cur, cur, p_)
+                else:
+                    cval = _flt(self.info[ins[1] if ins[0] == prev else ins[0]].value)
+                    e = _Emit._binary(op, cur, cval, p_) if ins[0] == prev else _Emit._binary(op, cval, cur, p_)
+                lines.append(f"    const float t{k} = {e};")
+                cur, prev = f"t{k}", nid
+            fns.append(f"__device__ __forceinline__ float xf{si}(const float x) {{\n" + "\n".join(lines) +
+                       f"\n    return {cur};\n}}\n")
+        return fns
 
     def _pick_edge_mode(self, mode: str) -> str:
         if mode in ("band", "variants"):
@@ -1303,6 +1382,7 @@ class KernelGen:
                 # one spare row per channel (the last channel's absorbs the stores of unused
                 # halo entries)
                 lines.append(f"__shared__ float lds_s{i}{dims};")
+        lines += self._xform_functions()
         fns = []
         for m in self._variants():
             name, fn = self._emit_variant(m)
@@ -1499,8 +1579,16 @@ class KernelGen:
             A(f"{indent}  for (int j = 0; j < {KH}; ++j) {buf}h{i}[j] = p.src{i}[pb + hg{i}[j]];")
         A(f"{indent}}}")
 
-    def _store(self, A, i: int, buf: str, slot: str, indent: str = "    ") -> None:
+    def _xf_own(self, A, i: int, buf: str, indent: str = "    ") -> None:
+        """Apply source i's load transform to its own-cell buffer in place."""
+        if i in self.xform:
+            A(f"{indent}#pragma unroll")
+            A(f"{indent}for (int k = 0; k < VPT; ++k) {buf}o{i}[k][0] = xf{i}({buf}o{i}[k][0]);")
+
+    def _store(self, A, i: int, buf: str, slot: str, indent: str = "    ", own_done: bool = False) -> None:
         L_ = self.src_layout[i]
+        if not own_done:
+            self._xf_own(A, i, buf, indent)
         if not L_["lds"]:
             return
         C, H, KH = L_["C"], L_["H"], L_.get("KH", 0)
@@ -1511,8 +1599,9 @@ class KernelGen:
         A(f"{indent}    #pragma unroll")
         A(f"{indent}    for (int c = 0; c < {C}; ++c) ls[(c * {RH + 1} + ly0 + k * RP + {H}) * {W} + lx + {H}] = {buf}o{i}[k][c];")
         if KH:
+            xf = (lambda v: f"xf{i}({v})") if i in self.xform else (lambda v: v)
             A(f"{indent}  #pragma unroll")
-            A(f"{indent}  for (int j = 0; j < {KH}; ++j) ls[hl{i}[j]] = {buf}h{i}[j];")
+            A(f"{indent}  for (int j = 0; j < {KH}; ++j) ls[hl{i}[j]] = {xf(f'{buf}h{i}[j]')};")
         A(f"{indent}}}")
 
     def _lead(self, i: int) -> int:
@@ -1523,10 +1612,12 @@ class KernelGen:
             C = L_["C"]
             if L_["mode"] == "queue":
                 self._fetch(A, i, "A", "zb - 1")
+                self._xf_own(A, i, "A")
                 A(f"    for (int k = 0; k < VPT; ++k) for (int c = 0; c < {C}; ++c) qm{i}[k][c] = Ao{i}[k][c];")
                 self._fetch(A, i, "A", "zb")
+                self._xf_own(A, i, "A")
                 A(f"    for (int k = 0; k < VPT; ++k) for (int c = 0; c < {C}; ++c) q0{i}[k][c] = Ao{i}[k][c];")
-                self._store(A, i, "A", "0")
+                self._store(A, i, "A", "0", own_done=True)
             else:
                 A("    for (int k = -HZ; k <= HZ; ++k) {")
                 self._fetch(A, i, "A", "zb + k", "        ")
@@ -1541,6 +1632,9 @@ class KernelGen:
         z = zname
         A(f"{ind}{{ const int s = {z} - zb + HZ;")
         A(f"{ind}  const bool ez = ({z} < REDGE) || ({z} + REDGE > N0 - 1);")
+        # queue sources: the next plane enters the register queue now, transformed
+        for i in self.queue_sources():
+            self._xf_own(A, i, buf, ind + "  ")
         if self.fw:
             A(f"{ind}  if (!fwave) {{")
         A(f"{ind}  #pragma unroll")
@@ -1585,7 +1679,7 @@ class KernelGen:
         if self.use_lds:
             for i, L_ in enumerate(self.src_layout):
                 slot = str(sq ^ 1) if L_["mode"] == "queue" else f"(unsigned)(s + HZ + 1) % {L_['D']}u"
-                self._store(A, i, buf, slot, ind + "  ")
+                self._store(A, i, buf, slot, ind + "  ", own_done=L_["mode"] == "queue")
             A(f"{ind}  __syncthreads();")
             for i in self.queue_sources():
                 C = self.src_layout[i]["C"]
@@ -1901,7 +1995,7 @@ class Launch:
             return
         main = torch.cuda.current_stream(dev) if stream is None else torch.cuda.ExternalStream(st, device=dev)
         self.side.wait_stream(main)
-        self.cg.kernel(entry).launch((self.nblocks, 1, 1), (gen.block_threads(), 1, 1), a, EXTRA_LDS, st)
+        self.cg.kernel(entry).launch((self.nblocks, 1, 1), (gen.block_threads(), 1, 1), a, 0, st)
         self.cg.kernel(tail).launch((nbt, 1, 1), (gen.block_threads(), 1, 1), a, 0, self.side.cuda_stream)
         main.wait_stream(self.side)
 

@@ -102,3 +102,14 @@ def test_main_and_boundary_kernels_cover_every_voxel_once(shape, sched, edge_mod
     assert y0 >= min(r, n1) * (edge_mode == "band") and x0 >= min(r, n2) * (edge_mode == "band")
     assert gen.boundary_blocks() * gen.block_threads() * gen.TAIL_VPT >= sum((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4])
                                               for b in gen.boundary_boxes())
+
+
+def test_load_transforms_detected_only_where_they_pay():
+    cases = {"laplacian(tanh(f))": ["tanh"], "laplacian(tanh(f)) * tanh(f)": ["tanh"],
+             "gradient(exp(f*2))": ["mul", "exp"], "hessian(log(f*f+1))": ["mul", "add", "log"],
+             "laplacian(f)": None, "tanh(f) + laplacian(f)": None, "laplacian(f*2)": None}
+    for expr, chain in cases.items():
+        g, sink, reds, steps, comp = plan(expr, bc="neumann")
+        gen = comp[0].gen
+        got = [g.node(n).op_name for n in gen.xform[0][0]] if gen.xform else None
+        assert got == chain, (expr, got)

@@ -155,3 +155,15 @@ def test_captured_fused_run_replays_match_eager():
             for k in wanted:
                 assert torch.equal(res[k], eager[k]), (rep, k)
             torch.randn(1 << 20, device="cuda").sum().item()     # unrelated eager work
+
+
+@pytest.mark.parametrize("edge_mode", ["band", "variants"])
+@pytest.mark.parametrize("bc", ["periodic", "neumann", "dirichlet"])
+def test_load_transformed_sources(bc, edge_mode):
+    """Point-wise chains with transcendentals under stencils are applied once per loaded cell
+    (queue and ring sources, partial tiles, every boundary): results stay within the rule."""
+    shape = (22, 27, 70)
+    f = field(shape, 12)
+    for expr in ["laplacian(tanh(f)) * tanh(f)", "divergence(gradient(tanh(f)))", "gradient(exp(f)) * 2",
+                 "hessian(log(f*f+1))", "laplacian(laplacian(exp(f*0.5)))"]:
+        run_and_check(expr, {"f": f}, boundary=bc, edge_mode=edge_mode)
