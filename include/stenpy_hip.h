@@ -90,6 +90,14 @@ int sp_op_normalize_last(const sp_tensor* a, sp_tensor* out, double eps, void* s
 enum { SP_D1_PERIODIC = 0, SP_D1_EDGE2 = 1, SP_D1_EDGE1 = 2, SP_D2_PAD = 3 };
 int sp_axis_pass(const sp_tensor* x, int axis, sp_tensor* out, int mode, int pad, double h,
                  double scale, int accumulate, void* stream);
+/* Separable filter pass: out[i] = sum_k w[k] * x[i + k - ntaps/2] along ``axis``, reads past
+ * the ends padded by ``pad`` (boundary enum, as the reference's _pad1d).  ``weights``: host
+ * array of ``ntaps`` (odd, <= SP_FIR_MAX_TAPS).  Building block of moving_average (box taps,
+ * stenpy_engine.py:2385-2408) and conv3d_spectral ((1, 2, 1) / 4 periodic taps: the
+ * reference's FFT convolution with the 27-point Gaussian, 2800-2846). */
+#define SP_FIR_MAX_TAPS 31
+int sp_axis_fir(const sp_tensor* x, int axis, sp_tensor* out, int ntaps, const double* weights,
+                int pad, void* stream);
 
 /* 3-D single-pass kernels for scalar (n0,n1,n2) / vector (n0,n1,n2,3) fields.
  * Each replaces the CPU branch of the cited reference op (the parity target). */

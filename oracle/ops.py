@@ -394,6 +394,19 @@ def moving_average(t: Tensor, window: int = 3, boundary: str = "neumann") -> Ten
     return out
 
 
+def conv3d_spectral(a: Tensor, boundary: str = "periodic", dx: Any = None) -> Tensor:
+    """FFT circular convolution with the 27-point (1, 2, 1)^3 / 64 kernel centred at the
+    origin (stenpy_engine.py:2823-2846, the CPU path: the boundary argument has no effect)."""
+    k = torch.tensor([[[1, 2, 1], [2, 4, 2], [1, 2, 1]],
+                      [[2, 4, 2], [4, 8, 4], [2, 4, 2]],
+                      [[1, 2, 1], [2, 4, 2], [1, 2, 1]]], dtype=a.dtype) / 64.0
+    k_full = torch.zeros(a.shape, dtype=a.dtype)
+    k_full[:3, :3, :3] = k
+    k_full = torch.roll(k_full, shifts=(-1, -1, -1), dims=(0, 1, 2))
+    c = torch.fft.ifftn(torch.fft.fftn(a) * torch.fft.fftn(k_full))
+    return c.real.to(a.dtype)
+
+
 def cumulative_integral(t: Tensor, dx: float = 1.0, dim: int = 0) -> Tensor:   # 2462-2483
     n = t.shape[dim]
     if n < 2:
@@ -449,7 +462,7 @@ OPS = {
     "scale_eye": scale_eye, "velocity_dot_grad": velocity_dot_grad,
     "dot": dot, "sqnorm": sqnorm,
     "sum": op_sum, "mean": mean, "norm_l2": norm_l2, "min_max": min_max,
-    "variance": variance, "integrate": integrate, "moving_average": moving_average,
+    "variance": variance, "integrate": integrate, "moving_average": moving_average, "conv3d_spectral": conv3d_spectral,
     "cumulative_integral": cumulative_integral,
     "trace": trace, "determinant": determinant, "eigenvalues": eigenvalues,
     "inverse": inverse, "deviatoric": deviatoric,

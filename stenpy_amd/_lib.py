@@ -73,6 +73,7 @@ def _declare(lib: C.CDLL) -> None:
         "sp_op_sqnorm": ([P, P, V], I),
         "sp_op_norm_last": ([P, P, D, V], I),
         "sp_op_normalize_last": ([P, P, D, V], I),
+        "sp_axis_fir": ([P, I, P, I, C.POINTER(D), I, V], I),
         "sp_axis_pass": ([P, I, P, I, I, D, D, I, V], I),
         "sp_laplacian3d": ([P, P, C.POINTER(D), I, V], I),
         "sp_gradient3d": ([P, P, D, I, V], I),
@@ -201,6 +202,14 @@ def axis_pass(x: torch.Tensor, axis: int, out: torch.Tensor, mode: int, pad: int
     require_gpu(x, out)
     check(lib().sp_axis_pass(C.byref(desc(x)), axis, C.byref(desc(out)), mode, pad, float(h),
                              float(scale), int(accumulate), stream_ptr(x.device)), "axis pass")
+    return out
+
+
+def axis_fir(x: torch.Tensor, axis: int, out: torch.Tensor, weights, pad: int) -> torch.Tensor:
+    require_gpu(x, out)
+    w = (C.c_double * len(weights))(*[float(v) for v in weights])
+    check(lib().sp_axis_fir(C.byref(desc(x)), axis, C.byref(desc(out)), len(weights), w, pad,
+                            stream_ptr(x.device)), "axis fir")
     return out
 
 

@@ -88,6 +88,52 @@ __global__ void __launch_bounds__(256) k_axis_pass(const float* __restrict__ x, 
     }
 }
 
+// ------------------------------------------------------------------------------ FIR pass
+struct FirTaps { float w[SP_FIR_MAX_TAPS]; };
+
+// Source index of padded coordinate q for a pad of radius r, as the reference's _pad1d fallback
+// loops fill it: the left side wraps / mirrors / replicates, while on the right side cell
+// n-1+k is filled from loop index r-k, so for r >= 2 a circular pad comes out mirrored
+// (src = r-k) and a reflect pad reads n-2-r+k (stenpy_engine.py:1395-1413).  false: zero.
+__device__ __forceinline__ bool fir_pad_index(int64_t q, int64_t n, int mode, int r, int64_t& src) {
+    if (q >= 0 && q < n) { src = q; return true; }
+    if (mode == SP_NEUMANN) { src = q < 0 ? 0 : n - 1; return true; }
+    if (mode == SP_PERIODIC) { src = q < 0 ? q + n : r - (q - (n - 1)); return src >= 0 && src < n; }
+    if (mode == SP_REFLECT && n > 1) {
+        if (q < 0) { src = -q < n - 1 ? -q : n - 1; return true; }
+        const int64_t v = n - 2 - r + (q - (n - 1));
+        src = v > 0 ? v : 0;
+        return true;
+    }
+    return false;
+}
+
+// out[i] = sum_k w[k] * x_pad[i + k - ntaps/2] along one axis, taps summed in order (the
+// order of the reference's conv1d cross-correlation); x_pad follows the padding mode.
+__global__ void __launch_bounds__(256) k_axis_fir(const float* __restrict__ x, int64_t outer,
+                                                  int64_t n, int64_t inner, int64_t xo, int64_t xn,
+                                                  int64_t xi, float* __restrict__ y, int64_t yo,
+                                                  int64_t yn, int64_t yi, int ntaps, int pad,
+                                                  const FirTaps taps) {
+    const int64_t total = outer * n * inner;
+    const int h = ntaps / 2;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t % inner;
+        const int64_t r = t / inner;
+        const int64_t i = r % n;
+        const int64_t o = r / n;
+        const float* base = x + o * xo + j * xi;
+        float acc = 0.0f;
+        for (int k = 0; k < ntaps; ++k) {
+            int64_t src;
+            const float v = fir_pad_index(i + k - h, n, pad, h, src) ? base[src * xn] : 0.0f;
+            acc += v * taps.w[k];
+        }
+        y[o * yo + i * yn + j * yi] = acc;
+    }
+}
+
 // ------------------------------------------------------------------------------ 3-D kernels
 #define SP_VOXEL_PROLOGUE(G)                                               \
     const int64_t px = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;    \
@@ -528,4 +574,32 @@ extern "C" int sp_material_derivative3d(const sp_tensor* x, const sp_tensor* v, 
                        (const float*)x->ptr, (const float*)v->ptr, (float*)out->ptr, g,
                        (float)dx[0], (float)dx[1], (float)dx[2]);
     return launch_status("sp_material_derivative3d");
+}
+
+extern "C" int sp_axis_fir(const sp_tensor* x, int axis, sp_tensor* out, int ntaps,
+                           const double* weights, int pad, void* stream) {
+    SP_REQUIRE(x && x->ptr && x->dtype == SP_F32, "axis fir: input must be f32");
+    SP_REQUIRE(out && out->ptr && out->dtype == SP_F32, "axis fir: output must be f32");
+    SP_REQUIRE(x->ndim == out->ndim, "axis fir: rank mismatch");
+    for (int d = 0; d < x->ndim; ++d)
+        SP_REQUIRE(x->shape[d] == out->shape[d], "axis fir: shape mismatch");
+    SP_REQUIRE(weights && ntaps >= 1 && ntaps <= SP_FIR_MAX_TAPS && (ntaps & 1),
+               "axis fir: odd tap count 1..SP_FIR_MAX_TAPS required");
+    SP_REQUIRE(pad >= SP_PERIODIC && pad <= SP_ZEROPAD, "axis fir: unknown padding");
+    int64_t xo, xn, xi, yo, yn, yi, outer, n, inner, o2, n2, i2;
+    int rc = collapse(x, axis, outer, n, inner, xo, xn, xi);
+    if (rc) return rc;
+    rc = collapse(out, axis, o2, n2, i2, yo, yn, yi);
+    if (rc) return rc;
+    SP_REQUIRE(pad != SP_PERIODIC || n >= ntaps / 2, "axis fir: periodic pad wider than the axis");
+    FirTaps taps;
+    for (int k = 0; k < SP_FIR_MAX_TAPS; ++k) taps.w[k] = k < ntaps ? (float)weights[k] : 0.0f;
+    const int64_t total = outer * n * inner;
+    if (total == 0) return SP_OK;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_axis_fir, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream),
+                       (const float*)x->ptr, outer, n, inner, xo, xn, xi, (float*)out->ptr, yo, yn,
+                       yi, ntaps, pad, taps);
+    return launch_status("sp_axis_fir");
 }

@@ -340,6 +340,45 @@ for _name, _fn, _r, _cost in [("laplacian", laplacian, 1, "high"), ("gradient", 
                       kind="stencil")
 
 
+# ----------------------------------------------------------------------------- separable filters
+def moving_average(t, window: int = 3, boundary: str = "neumann"):
+    """Box filter of width ``window`` along every axis, axis 0 first, each pass padded by
+    ``boundary`` (stenpy_engine.py:2385-2408).  An even window changes the length in the
+    reference's conv1d formulation and fails there; it is rejected here."""
+    dev = _device_of(t)
+    t = _contig(_f32(t, dev))
+    if t.ndim > 3:
+        raise NotImplementedError("moving_average: only 1-D/2-D/3-D supported")
+    window = int(window)
+    if window < 1 or window % 2 == 0:
+        raise ValueError(f"moving_average: window must be odd, got {window}")
+    w = [1.0 / window] * window
+    out = t
+    for d in range(t.ndim):
+        out = L.axis_fir(out, d, _empty(out.shape, dev), w, L.boundary_code(boundary))
+    return out
+
+
+def conv3d_spectral(a, boundary: str = "periodic", dx: Any = None):
+    """27-point Gaussian blur, (1, 2, 1)/4 along each axis with periodic wrap: the reference's
+    CPU path convolves through the FFT, which wraps whatever ``boundary`` says
+    (stenpy_engine.py:2800-2846).  ``dx`` (added by the compiler for stencil ops) is ignored."""
+    dev = _device_of(a)
+    a = _contig(_f32(a, dev))
+    if a.ndim != 3 or min(a.shape) < 3:
+        raise ValueError(f"conv3d_spectral: expected a 3-D field with every axis >= 3, got {tuple(a.shape)}")
+    out = a
+    for d in range(3):
+        out = L.axis_fir(out, d, _empty(out.shape, dev), [0.25, 0.5, 0.25], L.BOUNDARY["periodic"])
+    return out
+
+
+register_operator("moving_average", moving_average, radius=1, halo_l=1, halo_r=1, cost="medium",
+                  kind="stencil")
+register_operator("conv3d_spectral", conv3d_spectral, radius=1, halo_l=1, halo_r=1, cost="very_high",
+                  flops=2, bytes_in=8, bytes_out=8, kind="stencil")
+
+
 # ----------------------------------------------------------------------------- channel-last helpers
 def stack_components(*tensors, dim: int = -1):
     dev = _device_of(*tensors)

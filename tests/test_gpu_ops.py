@@ -201,3 +201,29 @@ def test_eigenvalues_degenerate_branches():
     A = 0.5 * (A + A.transpose(1, 2))
     o64, o32 = ref(O.eigenvalues, A)
     assert_2x(ops.eigenvalues(A.cuda()), o64, o32, "eig near-degenerate")
+
+
+@pytest.mark.parametrize("bc", ["periodic", "neumann", "dirichlet", "reflect"])
+@pytest.mark.parametrize("window", [3, 5])
+def test_moving_average(bc, window):
+    f = field((18, 22, 40), 21)
+    out = ops.moving_average(f, window, bc)
+    f64 = f.double().cpu()
+    assert_2x(out, O.moving_average(f64, window, bc), O.moving_average(f64.float(), window, bc),
+              f"moving_average {bc} w={window}")
+    with pytest.raises(ValueError):
+        ops.moving_average(f, 4, bc)
+
+
+def test_conv3d_spectral_and_expressions():
+    f = field((18, 22, 40), 22)
+    out = ops.conv3d_spectral(f)
+    f64 = f.double().cpu()
+    assert_2x(out, O.conv3d_spectral(f64), O.conv3d_spectral(f64.float()), "conv3d_spectral")
+    # both are reachable from expressions (eager plain kernels inside a fused graph)
+    from oracle import graph_eval
+    for expr in ["moving_average(f) + laplacian(f)", "conv3d_spectral(tanh(f)) * 2"]:
+        g, sink, *_ = sp.compile_expression(expr, boundary="periodic", field_map={"f": f})
+        rt = sp.Runtime(device="cuda")
+        res = rt.run(g)[sink]
+        assert_2x(res, graph_eval.evaluate(g, torch.float64)[sink], graph_eval.evaluate(g, torch.float32)[sink], expr)

@@ -342,3 +342,69 @@ def test_fp32_twin_tracks_fp64():
         assert r32.dtype == torch.float32
         err = (r32.double() - r64).abs().max().item()
         assert 0 < err < 1e-5 * max(1.0, r64.abs().max().item())
+
+
+# ------------------------------------------------------------------ separable filters
+@pytest.mark.parametrize("bc", ["periodic", "neumann", "dirichlet"])
+@pytest.mark.parametrize("window", [1, 3, 5])
+def test_moving_average_brute_force(bc, window):
+    """Separable box passes (axis 0, then 1, then 2) with _pad1d's fallback-loop padding
+    (whose circular right side is mirrored for radius >= 2), as nested loops."""
+    shape = (5, 6, 7)
+    a = rand(shape)
+    h = window // 2
+
+    def pad_map(n, r):
+        """Padded position -> source index, filled exactly as the reference's fallback loops
+        do (stenpy_engine.py:1395-1413); None = zero."""
+        m = [None] * r + list(range(n)) + [None] * r
+        for i in range(r):
+            if bc == "neumann":
+                m[i], m[-(i + 1)] = 0, n - 1
+            elif bc == "periodic":
+                m[i], m[-(i + 1)] = n - (r - i), i
+        return m
+
+    def padded(arr, d, i):
+        n = arr.shape[d]
+        m = pad_map(n, h)
+        src = m[i[d] + h]
+        if src is None:
+            return 0.0
+        j = list(i)
+        j[d] = src
+        return float(arr[tuple(j)])
+
+    cur = a.clone()
+    for d in range(3):
+        nxt = torch.zeros_like(cur)
+        for i in np.ndindex(*shape):
+            acc = 0.0
+            for k in range(-h, h + 1):
+                j = list(i)
+                j[d] += k
+                acc += padded(cur, d, tuple(j)) / window
+            nxt[i] = acc
+        cur = nxt
+    torch.testing.assert_close(O.moving_average(a, window, bc), cur, rtol=1e-13, atol=1e-13)
+    # a constant is a fixed point except where zero padding enters
+    if bc != "dirichlet":
+        c = torch.full(shape, 2.5, dtype=torch.float64)
+        torch.testing.assert_close(O.moving_average(c, window, bc), c, rtol=1e-14, atol=1e-14)
+
+
+def test_conv3d_spectral_is_the_periodic_27_point_blur():
+    shape = (4, 5, 6)
+    a = rand(shape)
+    w = {-1: 0.25, 0: 0.5, 1: 0.25}
+    exp = bf_grid(lambda i: sum(w[p] * w[q] * w[r] * float(a[(i[0] + p) % 4, (i[1] + q) % 5, (i[2] + r) % 6])
+                                for p in (-1, 0, 1) for q in (-1, 0, 1) for r in (-1, 0, 1)), shape)
+    torch.testing.assert_close(O.conv3d_spectral(a, boundary="neumann"), exp, rtol=1e-12, atol=1e-12)
+    # Fourier symbol of a single mode: prod_d (1 + cos theta_d) / 2
+    n = (8, 10, 12)
+    k = (1, 3, 2)
+    z, y, x = torch.meshgrid(*[torch.arange(m, dtype=torch.float64) for m in n], indexing="ij")
+    th = [2 * math.pi * k[d] / n[d] for d in range(3)]
+    f = torch.cos(th[0] * z + th[1] * y + th[2] * x)
+    sym = math.prod((1 + math.cos(t)) / 2 for t in th)
+    torch.testing.assert_close(O.conv3d_spectral(f), sym * f, rtol=1e-12, atol=1e-12)
