@@ -1,4 +1,4 @@
-"""Benchmark: config-2 graph on a 512^3 grid, fused/autotuned vs plain per-operator HIP.
+"""Benchmark: the 512^3 stencil graph, fused/autotuned vs plain per-operator HIP.
 
 One step = one evaluation of
     out = divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))
@@ -22,14 +22,14 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-C2_EXPR = "divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))"
+STENCIL_GRAPH = "divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))"
 REDUCTIONS = ("sum", "mean", "variance", "norm_l2")
 HBM_PEAK_GBS = 8000.0
 
 
 def build_graph(f, u):
     import stenpy_amd as sp
-    g, sink, _, _, _ = sp.compile_expression(C2_EXPR, dx=1.0, boundary="periodic",
+    g, sink, _, _, _ = sp.compile_expression(STENCIL_GRAPH, dx=1.0, boundary="periodic",
                                              field_map={"f": f, "u": u})
     reds = [g.add(r, (sink,), {}) for r in REDUCTIONS]
     return g, sink, reds
@@ -112,7 +112,7 @@ def _line(metric, value, unit, steps, warmup, ms, config, **extra):
     print(json.dumps(d), flush=True)
 
 
-def bench_c3(args) -> int:
+def bench_batched(args) -> int:
     """96 fields, axes U{32..384} (seed 6), 6 random graphs of 3-12 ops (seed 6); field i uses
     graph i mod 6 and seed 1000 + i; every level one persistent launch, replayed as a hipGraph."""
     import random
@@ -145,7 +145,7 @@ def bench_c3(args) -> int:
     seq = lambda: [rt.fused.run(g, rt, wanted=w) for g, w in items]
     t_seq = time_steps(seq, max(2, args.steps // 5), 1)
     vox = st["voxels"]
-    _line("C3 batched persistent launch: Gvoxel/s over 96 uneven fields x 6 random graphs",
+    _line("batched persistent launch: Gvoxel/s over 96 uneven grids x 6 random graphs",
           round(vox / t / 1e9, 2), "Gvoxel/s", args.steps, args.warmup, t * 1e3,
           {"workload": "96 fields, axes U{32..384} seed 6; 6 random graphs (3-12 ops) seed 6; periodic",
            "exprs": exprs, "levels": st["levels"], "units": st["units"], "structures": st["structures"],
@@ -158,7 +158,7 @@ def bench_c3(args) -> int:
     return 0
 
 
-def bench_c5(args) -> int:
+def bench_tensor(args) -> int:
     """768^3: lower half seeded Fourier (seed 8), upper half i^2 + j^2 + k^2; hessian ->
     eigenvalues, determinant, trace, deviatoric, mean curvature, fused."""
     import torch
@@ -189,7 +189,7 @@ def bench_c5(args) -> int:
     t = time_steps(step, max(3, args.steps // 4), 1)
     nvox = n ** 3
     alg = nvox * 4 * (1 + 3 + 1 + 1 + 9 + 1)     # read f once, write 15 output channels (64 B)
-    _line("C5 tensor ops: Gvoxel/s of hessian -> eig/det/trace/dev + mean curvature on 768^3",
+    _line("tensor ops: Gvoxel/s of hessian -> eig/det/trace/dev + mean curvature on 768^3",
           round(nvox / t / 1e9, 3), "Gvoxel/s", max(3, args.steps // 4), 1, t * 1e3,
           {"workload": f"{n}^3 FP32, lower half Fourier seed 8, upper half i^2+j^2+k^2; periodic; "
                        "one graph with five sinks sharing the hessian", "exprs": exprs,
@@ -200,7 +200,7 @@ def bench_c5(args) -> int:
     return 0
 
 
-def bench_c4(args) -> int:
+def bench_stream(args) -> int:
     """4096 x 4096 x 4608 field generated on the host slab by slab (256 planes, 1-plane halo),
     streamed through pinned buffers; a bounded sample of ``--slabs`` slabs is timed."""
     import torch
@@ -217,7 +217,7 @@ def bench_c4(args) -> int:
     st = stream_graph(g, sink, fm["f"], lazy, None, rt, boundary="periodic", slab_planes=256,
                       max_slabs=args.slabs)
     vox = st["voxels"]
-    _line("C4 out-of-core stream: Gvoxel/s (host generation + H2D + fused compute), bounded sample",
+    _line("out-of-core stream: Gvoxel/s (host generation + H2D + fused compute), bounded sample",
           round(st["gvoxel_s"], 3), "Gvoxel/s", 1, 0, st["wall_s"] * 1e3,
           {"workload": "4608x4096x4096 FP32 seed 7, 256-plane slabs, periodic halo 1; "
                        f"sample = first {st['n_chunks']} slabs", "slabs": st["n_chunks"],
@@ -231,7 +231,7 @@ def copy_bytes(n: int) -> int:
 
 
 def profile_pass(args) -> int:
-    """Fused C2 launches only (plus one calibration copy): what rocprofv3 measures."""
+    """Fused stencil-graph launches only (plus one calibration copy): what rocprofv3 measures."""
     import torch
     import stenpy_amd as sp
     from stenpy_amd import _lib as L, gen, tune
@@ -289,10 +289,10 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-planes", type=int, default=32)
-    ap.add_argument("--traffic", type=str, default=str(ROOT / "profiles" / "traffic_c2_fused_512.json"),
+    ap.add_argument("--traffic", type=str, default=str(ROOT / "profiles" / "traffic_stencil_graph_512.json"),
                     help="JSON file with measured HBM bytes per launch of the fused kernel")
     ap.add_argument("--profile-pass", action="store_true",
-                    help="only launch the fused C2 kernel --steps times plus a streaming copy of known "
+                    help="only launch the fused stencil-graph kernels --steps times plus a streaming copy of known "
                          "size (the target of rocprofv3 --pmc / --kernel-trace runs)")
     ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None,
                     help="turn two rocprofv3 --pmc CSV outputs into the traffic JSON (no GPU)")
@@ -300,11 +300,12 @@ def main() -> int:
                     help="render a rocprofv3 --stats CSV output as markdown (no GPU)")
     ap.add_argument("--cmd", type=str, default="", help="command line quoted in --stats-md output")
     ap.add_argument("--out", type=str, default=None)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2",
-                    help="c2: the headline 512^3 graph (default); c3: 96 uneven fields x 6 random "
-                         "graphs in batched persistent launches; c4: out-of-core stream (bounded "
-                         "sample of slabs); c5: 768^3 hessian tensor ops")
-    ap.add_argument("--slabs", type=int, default=2, help="c4: slabs of 256 planes to stream")
+    ap.add_argument("--workload", choices=["graph", "batched", "stream", "tensor"], default="graph",
+                    help="graph: the 512^3 stencil graph (default); batched: 96 uneven grids x 6 "
+                         "random graphs in batched persistent launches; stream: out-of-core "
+                         "4608x4096x4096 stream (bounded sample of slabs); tensor: 768^3 hessian "
+                         "tensor ops")
+    ap.add_argument("--slabs", type=int, default=2, help="stream: slabs of 256 planes to stream")
     args = ap.parse_args()
     if args.pmc_traffic or args.stats_md:
         return offline_reports(args)
@@ -313,8 +314,8 @@ def main() -> int:
         return 2
     if args.profile_pass:
         return profile_pass(args)
-    if args.workload != "c2":
-        return {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)
+    if args.workload != "graph":
+        return {"batched": bench_batched, "stream": bench_stream, "tensor": bench_tensor}[args.workload](args)
 
     import torch
     import stenpy_amd as sp
@@ -370,7 +371,7 @@ def main() -> int:
         "ms_per_step": round(t_fused * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded 8-mode Fourier fields + counter-based noise)",
-        "config": {"workload": f"config-2 graph, {n}^3 FP32 f + {n}^3x3 u, periodic, dx=1; "
+        "config": {"workload": f"stencil graph lap(f) + u.grad(f) + |curl u|^2, {n}^3 FP32 f + {n}^3x3 u, periodic, dx=1; "
                                "out + sum/mean/variance/L2",
                    "schedule": list(cg.gen.sched.as_tuple()),
                    "hbm_frac_step": round(20 * nvox / t_fused / 1e9 / HBM_PEAK_GBS, 4),

@@ -17,7 +17,7 @@ Behaviour follows the reference compiler (stenpy_engine.py:3458-3778):
   hash-consed on ``(op, child ids, scalar params)``.
 
 Deliberate differences: ``dot`` and ``sqnorm`` are extra point-wise op names (the reference
-cannot express config-2's ``u . grad f`` and ``|curl u|^2``), and dead nodes (such as the
+cannot express ``u . grad f`` and ``|curl u|^2``), and dead nodes (such as the
 gradient the divergence(gradient) rewrite leaves behind) are removed after lowering.
 """
 from __future__ import annotations

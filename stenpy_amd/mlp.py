@@ -1,7 +1,7 @@
 """Learned schedule model: a 4-layer width-256 MLP trained on autotuner timings.
 
 The network maps (group features, schedule features) to the log of the kernel time per
-voxel; the autotuner's exhaustive search stays the fallback and the judge of its regret.
+voxel; the autotuner's exhaustive search stays the fallback and the measure of its regret.
 Weights are FP32 masters with BF16 copies; every matmul is ``sp_gemm_bf16`` (MFMA, FP32
 accumulate) and every other step (bias + ReLU, ReLU backward, bias gradients, MSE, Adam) is
 a HIP kernel of libstenpy_hip -- torch only allocates the buffers.

@@ -6,7 +6,7 @@ semantics, and computes on the GPU through the C ABI.  Deliberate contract chang
 * storage and arithmetic are FP32 (FP64 / BF16 inputs are converted on entry; outputs are
   FP32, reductions return FP64 0-d tensors); the reference asserts FP64 everywhere;
 * inputs must be on a HIP device -- there is no CPU path;
-* ``dot`` and ``sqnorm`` are added (config-2 graphs need them).
+* ``dot`` and ``sqnorm`` are added (u . grad f and |curl u|^2 need them).
 
 Pure data movement (stacking, slicing, reshaping) uses torch's device copies; every
 arithmetic step is a hand-written kernel.

@@ -1,4 +1,4 @@
-"""Random type-correct expression graphs (config-3 workloads and schedule-model training data).
+"""Random type-correct expression graphs (batched workloads and schedule-model training data).
 
 Ops are drawn from {gradient, laplacian, curl, divergence, exp, tanh, clamp, mul, add, mean}
 with scalar/vector typing: gradient S->V, curl V->V, divergence V->S, laplacian S->S,

@@ -13,7 +13,7 @@ evaluated on each padded slab and cropped, min/max/mean of the sink reported.  D
   next slab overlap the fused compute of the current one and the pinned footprint stays a
   few GiB even for 17 GB slabs;
 * a producer or writer exception is re-raised in the caller (the reference's reader thread
-  can deadlock the pipeline, SURVEY Appendix B item 12);
+  can deadlock the pipeline on an exception, stenpy_engine.py:773-816);
 * sink reductions (sum of squares for the L2 norm, sum, min, max) are taken over the cropped
   slab and combined across slabs in FP64.
 """

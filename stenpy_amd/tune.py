@@ -257,7 +257,7 @@ def default_time(rows: List[Dict[str, Any]], sched: Optional[Schedule] = None) -
 def random_group_rows(n_graphs: int, seed: int, runtime, cands: Sequence[Schedule] = None,
                       shape_range: Tuple[int, int] = (64, 256), reps: int = 5, workers: int = 12,
                       log=None) -> List[Dict[str, Any]]:
-    """Exhaustive timings of random config-3-style graphs on random grids (training data)."""
+    """Exhaustive timings of random op graphs (3-12 ops) on random grids (training data)."""
     import random
     from . import randgraph
     from .compiler import compile_expression

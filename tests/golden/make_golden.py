@@ -20,13 +20,13 @@ from oracle import graph_eval  # noqa: E402
 
 OUT = Path(__file__).resolve().parent
 EXPRS = {
-    "c1_pointwise": "f**2 + tanh(f)*exp(f)",
-    "c1_laplacian": "laplacian(f)",
-    "c1_gradient": "gradient(f)",
-    "c2_graph": "divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))",
-    "c5_eig": "eigenvalues(hessian(f))",
-    "c5_det": "determinant(hessian(f))",
-    "c5_meancurv": "mean_curvature(f)",
+    "pointwise": "f**2 + tanh(f)*exp(f)",
+    "laplacian": "laplacian(f)",
+    "gradient": "gradient(f)",
+    "stencil_graph": "divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))",
+    "hessian_eig": "eigenvalues(hessian(f))",
+    "hessian_det": "determinant(hessian(f))",
+    "mean_curvature": "mean_curvature(f)",
 }
 
 
@@ -38,7 +38,7 @@ def main():
     for name, expr in EXPRS.items():
         g, sink, *_ = sp.compile_expression(expr, boundary="periodic",
                                             field_map={"f": torch.from_numpy(f), "u": torch.from_numpy(u)})
-        reds = [g.add(r, (sink,), {}) for r in ("sum", "mean", "variance", "norm_l2")] if name == "c2_graph" else []
+        reds = [g.add(r, (sink,), {}) for r in ("sum", "mean", "variance", "norm_l2")] if name == "stencil_graph" else []
         o64 = graph_eval.evaluate(g, torch.float64)
         o32 = graph_eval.evaluate(g, torch.float32)
         arrays[f"{name}__out64"] = o64[sink].numpy()

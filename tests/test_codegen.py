@@ -6,7 +6,7 @@ import torch
 import stenpy_amd as sp
 from stenpy_amd import fused as F
 
-C2 = "divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))"
+STENCIL_GRAPH = "divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))"
 SHAPE = (20, 24, 30)
 
 
@@ -20,8 +20,8 @@ def plan(expr, bc="periodic", extra=(), shape=SHAPE):
     return g, sink, reds, steps, comp
 
 
-def test_c2_is_one_kernel_with_fused_reductions():
-    g, sink, reds, steps, comp = plan(C2, extra=("sum", "mean", "variance", "norm_l2"))
+def test_stencil_graph_is_one_kernel_with_fused_reductions():
+    g, sink, reds, steps, comp = plan(STENCIL_GRAPH, extra=("sum", "mean", "variance", "norm_l2"))
     assert [s.kind for s in steps] == ["group"]
     grp = steps[0].group
     assert grp.outputs == [sink] and sorted(grp.reductions) == sorted(reds)
@@ -52,7 +52,7 @@ def test_deep_composition_is_cut():
 
 
 @pytest.mark.parametrize("expr,bc", [
-    ("f**2 + tanh(f)*exp(f)", "periodic"), ("laplacian(f)", "periodic"), (C2, "periodic"),
+    ("f**2 + tanh(f)*exp(f)", "periodic"), ("laplacian(f)", "periodic"), (STENCIL_GRAPH, "periodic"),
     ("laplacian(laplacian(f))", "neumann"), ("divergence(curl(u)) + laplacian(exp(f))", "dirichlet"),
     ("gradient(f) * 2 + curl(u)", "reflect"), ("eigenvalues(hessian(f))", "periodic"),
     ("determinant(hessian(f)) + trace(hessian(f))", "neumann"), ("mean_curvature(f)", "periodic"),
@@ -75,7 +75,7 @@ def test_generated_kernels_compile(expr, bc):
 def test_main_and_boundary_kernels_cover_every_voxel_once(shape, sched, edge_mode):
     """Voxels the main kernel evaluates plus the boundary boxes tile the grid exactly."""
     f, u = torch.zeros(shape), torch.zeros(shape + (3,))
-    g, sink, *_ = sp.compile_expression(C2, boundary="neumann", field_map={"f": f, "u": u})
+    g, sink, *_ = sp.compile_expression(STENCIL_GRAPH, boundary="neumann", field_map={"f": f, "u": u})
     values = {n: g.node(n).params["value"] for n in g._order if g.node(n).op_name == "_constant"}
     steps, info = F.plan_graph(g, values, [sink])
     gen = F.KernelGen(g, steps[0].group, info, sched, {}, {}, edge_mode)

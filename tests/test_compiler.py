@@ -1,7 +1,7 @@
 """SymPy -> Graph compiler and Graph IR (CPU).
 
 Node lists follow the reference compiler's lowering rules (stenpy_engine.py:3618-3759) for
-the configuration expressions, with SymPy 1.14's canonical argument order.
+the benchmark expressions, with SymPy 1.14's canonical argument order.
 """
 import pytest
 import sympy

@@ -12,7 +12,7 @@ from tests._util import assert_2x, errs, field, vfield
 pytestmark = pytest.mark.gpu
 os.environ["STENPY_FUSED_STRICT"] = "1"
 
-C2_EXPR = "divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))"
+STENCIL_GRAPH = "divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))"
 
 
 def run_and_check(expr, fields, boundary="periodic", dx=1.0, reductions=(), schedule=None,
@@ -36,7 +36,7 @@ def run_and_check(expr, fields, boundary="periodic", dx=1.0, reductions=(), sche
     return res, out
 
 
-def test_c1_expressions():
+def test_single_op_expressions():
     f = field((32, 32, 32), 1)
     for expr in ["f**2 + tanh(f)*exp(f)", "laplacian(f)", "gradient(f)", "mean(f)", "norm_l2(f)"]:
         run_and_check(expr, {"f": f})
@@ -44,24 +44,24 @@ def test_c1_expressions():
 
 @pytest.mark.parametrize("edge_mode", ["band", "variants"])
 @pytest.mark.parametrize("shape", [(32, 32, 32), (21, 37, 70), (9, 8, 130)])
-def test_c2_graph(shape, edge_mode):
+def test_stencil_graph(shape, edge_mode):
     f = field(shape, 2)
     u = vfield(shape)
-    run_and_check(C2_EXPR, {"f": f, "u": u}, reductions=("sum", "mean", "variance", "norm_l2"),
+    run_and_check(STENCIL_GRAPH, {"f": f, "u": u}, reductions=("sum", "mean", "variance", "norm_l2"),
                   edge_mode=edge_mode)
 
 
 @pytest.mark.parametrize("sched", [Schedule(32, 8, 16, 256), Schedule(64, 8, 64, 256),
                                    Schedule(128, 4, 32, 256), Schedule(64, 8, 8, 512)])
-def test_c2_schedules(sched):
+def test_stencil_graph_schedules(sched):
     shape = (40, 45, 150)
-    run_and_check(C2_EXPR, {"f": field(shape, 2), "u": vfield(shape)}, schedule=sched,
+    run_and_check(STENCIL_GRAPH, {"f": field(shape, 2), "u": vfield(shape)}, schedule=sched,
                   reductions=("sum", "variance"))
 
 
-def test_c2_bf16_output():
+def test_stencil_graph_bf16_output():
     shape = (24, 30, 66)
-    run_and_check(C2_EXPR, {"f": field(shape, 2), "u": vfield(shape)},
+    run_and_check(STENCIL_GRAPH, {"f": field(shape, 2), "u": vfield(shape)},
                   reductions=("sum", "mean", "variance", "norm_l2"), out_dtype=torch.bfloat16)
 
 
@@ -78,7 +78,7 @@ def test_stencil_compositions(bc, edge_mode):
 
 
 @pytest.mark.parametrize("edge_mode", ["band", "variants"])
-def test_config5_tensor_ops(edge_mode):
+def test_hessian_tensor_ops(edge_mode):
     shape = (20, 24, 30)
     f = field(shape, 8)
     for expr in ["eigenvalues(hessian(f))", "determinant(hessian(f))", "trace(hessian(f))",
@@ -95,7 +95,7 @@ def test_scalar_from_reduction_feeds_next_group():
 def test_fused_matches_eager_path():
     shape = (20, 22, 68)
     f, u = field(shape, 2), vfield(shape)
-    g, sink, _, _, _ = sp.compile_expression(C2_EXPR, boundary="periodic", field_map={"f": f, "u": u})
+    g, sink, _, _, _ = sp.compile_expression(STENCIL_GRAPH, boundary="periodic", field_map={"f": f, "u": u})
     rt = sp.Runtime(device="cuda")
     a = rt.fused.run(g, rt, wanted=[sink])[sink]
     b = rt.run_eager(g)[sink]
@@ -138,7 +138,7 @@ def test_captured_fused_run_replays_match_eager():
     determinant flag pass): every replay reproduces the eager result exactly."""
     shape = (40, 45, 150)
     f, u = field(shape, 2), vfield(shape)
-    g, sink, _, _, _ = sp.compile_expression(C2_EXPR, boundary="periodic", field_map={"f": f, "u": u})
+    g, sink, _, _, _ = sp.compile_expression(STENCIL_GRAPH, boundary="periodic", field_map={"f": f, "u": u})
     reds = [g.add(r, (sink,), {}) for r in ("sum", "variance")]
     g2, sink2, _, _, _ = sp.compile_expression("determinant(hessian(f)) + trace(hessian(f))",
                                                boundary="periodic", field_map={"f": f})

@@ -2,7 +2,7 @@
 
 * exhaustive search picks the fastest candidate and records it in the database;
 * the schedule MLP, trained by the project's HIP kernels on the committed timing rows of
-  random graphs, is judged on held-out graphs: regret against exhaustive search and the gain
+  random graphs, is evaluated on held-out graphs: regret against exhaustive search and the gain
   of its pick over the shape heuristic used without it.
 """
 import json

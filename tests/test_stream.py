@@ -58,7 +58,7 @@ def test_npy_lazy_field(tmp_path):
     np.testing.assert_array_equal(out, a[4:7])
 
 
-C4 = "f**2 + tanh(f)*exp(f) + laplacian(f)"
+STREAM_EXPR = "f**2 + tanh(f)*exp(f) + laplacian(f)"
 
 
 @pytest.mark.gpu
@@ -68,10 +68,10 @@ def test_streamed_equals_whole_field(slab, stage_mb, tmp_path):
     shape = (40, 36, 70)
     lf = GeneratorField(shape, 7)
     whole = torch.from_numpy(lf[0:40]).cuda()
-    g, sink, _, fm, _ = sp.compile_expression(C4, boundary="periodic", field_map={"f": whole})
+    g, sink, _, fm, _ = sp.compile_expression(STREAM_EXPR, boundary="periodic", field_map={"f": whole})
     ref = sp.Runtime(device="cuda").run(g)[sink].cpu()
     probe = torch.zeros((1,) + shape[1:])
-    g2, sink2, _, fm2, _ = sp.compile_expression(C4, boundary="periodic", field_map={"f": probe})
+    g2, sink2, _, fm2, _ = sp.compile_expression(STREAM_EXPR, boundary="periodic", field_map={"f": probe})
     outp = str(tmp_path / "out.npy")
     st = stream_graph(g2, sink2, fm2["f"], lf, out_path=outp, slab_planes=slab, stage_mb=stage_mb)
     got = torch.from_numpy(np.load(outp))
@@ -92,6 +92,6 @@ def test_producer_error_propagates():
 
     lf = Broken((40, 16, 16), 1)
     probe = torch.zeros((1, 16, 16))
-    g, sink, _, fm, _ = sp.compile_expression(C4, boundary="periodic", field_map={"f": probe})
+    g, sink, _, fm, _ = sp.compile_expression(STREAM_EXPR, boundary="periodic", field_map={"f": probe})
     with pytest.raises(RuntimeError, match="disk on fire"):
         stream_graph(g, sink, fm["f"], lf, slab_planes=8)
