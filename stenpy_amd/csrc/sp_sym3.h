@@ -43,44 +43,52 @@ __device__ __forceinline__ void sort3(float* e) {
     if (e[0] > e[1]) { t = e[0]; e[0] = e[1]; e[1] = t; }
 }
 
-// Closed-form (trigonometric) eigenvalues of a symmetric 3x3 matrix.  With q = tr/3,
-// B = (A - qI)/p, p = sqrt(tr((A-qI)^2)/6) and r = det(B)/2 in [-1, 1], the eigenvalues are
-// q + 2p cos(acos(r)/3 + 2 pi k/3).  Near a double eigenvalue (|r| -> 1) acos loses
-// sqrt(eps) accuracy, so there the FP32 evaluation is replaced by FP64; a diagonal matrix
-// returns its sorted diagonal exactly.
+// Eigenvalues of a symmetric 3x3 (lower triangle, ascending, as eigvalsh), trigonometric closed
+// form: q = tr/3, p = sqrt(|A - qI|^2 / 6), r = det((A - qI)/p) / 2, eigenvalues
+// q + 2p cos(phi/3 + 2k pi/3) with phi = acos(r).  The angle is taken as
+// acos(r) = 2 atan2(sqrt(1 - r), sqrt(1 + r)), accurate where acos(r) itself is not (|r| -> 1).
+// Near a double eigenvalue (1 - |r| < SP_SYM3_FP64_BAND) r formed in FP32 has lost the digits
+// that matter, so there the moments q, p, r and the final combination are recomputed in FP64
+// (products of the FP32 entries); the trigonometry is FP32 in both branches.  A diagonal
+// matrix returns its sorted diagonal exactly.
 template <typename T>
-__device__ __forceinline__ void sym3_trig(T a00, T a11, T a22, T a10, T a20, T a21, T* e, T* r_out) {
-    const T q = (a00 + a11 + a22) / T(3);
+__device__ __forceinline__ void sym3_moments(const float* m, T& q, T& p, T& r) {
+    const T a00 = m[0], a11 = m[4], a22 = m[8];
+    const T a10 = m[3], a20 = m[6], a21 = m[7];
+    q = (a00 + a11 + a22) / T(3);
     const T b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
     const T p1 = a10 * a10 + a20 * a20 + a21 * a21;
     const T p2 = b00 * b00 + b11 * b11 + b22 * b22 + T(2) * p1;
-    const T p = sqrt(p2 / T(6));
+    p = sqrt(p2 / T(6));
     const T detB = b00 * (b11 * b22 - a21 * a21) - a10 * (a10 * b22 - a21 * a20) +
                    a20 * (a10 * a21 - b11 * a20);
-    T r = p > T(0) ? detB / (T(2) * p * p * p) : T(0);
+    r = p > T(0) ? detB / (T(2) * p * p * p) : T(0);
     r = r < T(-1) ? T(-1) : (r > T(1) ? T(1) : r);
-    const T phi = acos(r) / T(3);
-    const T two_pi_3 = T(2.0943951023931954923);
-    e[2] = q + T(2) * p * cos(phi);
-    e[0] = q + T(2) * p * cos(phi + two_pi_3);
-    e[1] = T(3) * q - e[0] - e[2];
-    *r_out = r;
 }
-
+template <typename T>
+__device__ __forceinline__ void sym3_from_moments(T q, T p, T r, float* e) {
+    const float sm = sqrtf((float)(T(1) - r)), sp = sqrtf((float)(T(1) + r));
+    const float phi = 2.0f * atan2f(sm, sp) / 3.0f;
+    const T c2 = (T)cosf(phi), c0 = (T)cosf(phi + 2.0943951023931954923f);
+    const T e2 = q + T(2) * p * c2, e0 = q + T(2) * p * c0;
+    e[0] = (float)e0;
+    e[1] = (float)(T(3) * q - e0 - e2);
+    e[2] = (float)e2;
+}
 __device__ __forceinline__ void sym3_eigvalsh(const float* m, float* e) {
-    const float a00 = m[0], a11 = m[4], a22 = m[8];
-    const float a10 = m[3], a20 = m[6], a21 = m[7];  // lower triangle, as eigvalsh
-    if (a10 == 0.0f && a20 == 0.0f && a21 == 0.0f) {
-        e[0] = a00; e[1] = a11; e[2] = a22;
+    if (m[3] == 0.0f && m[6] == 0.0f && m[7] == 0.0f) {
+        e[0] = m[0]; e[1] = m[4]; e[2] = m[8];
         sort3(e);
         return;
     }
-    float r;
-    sym3_trig<float>(a00, a11, a22, a10, a20, a21, e, &r);
+    float q, p, r;
+    sym3_moments<float>(m, q, p, r);
     if (1.0f - fabsf(r) < SP_SYM3_FP64_BAND || !(r == r)) {
-        double ed[3], rd;
-        sym3_trig<double>(a00, a11, a22, a10, a20, a21, ed, &rd);
-        e[0] = (float)ed[0]; e[1] = (float)ed[1]; e[2] = (float)ed[2];
+        double qd, pd, rd;
+        sym3_moments<double>(m, qd, pd, rd);
+        sym3_from_moments<double>(qd, pd, rd, e);
+    } else {
+        sym3_from_moments<float>(q, p, r, e);
     }
     sort3(e);
 }
