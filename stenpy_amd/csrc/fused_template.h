@@ -41,6 +41,13 @@ __device__ __forceinline__ int lmap(int c, int t0, int T, int H, int n) {
     return l + H;
 }
 
+// Orders one wave's LDS accesses across its lanes (a cross-lane exchange through LDS).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---- stores -----------------------------------------------------------------------------------
 __device__ __forceinline__ unsigned short f2bf(float f) {
     unsigned int u = __float_as_uint(f);

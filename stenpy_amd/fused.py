@@ -112,6 +112,8 @@ EDGE_MODE = os.environ.get("STENPY_EDGE_MODE", "auto")
 # separate small kernel (no LDS arrays, few registers) on a side stream, beside the LDS
 # z-marching kernel; off: the same work as extra blocks at the end of the main grid.
 SPLIT_TAIL = os.environ.get("STENPY_SPLIT_TAIL", "1") == "1"
+# stage wide channel-last outputs through LDS so every store instruction writes whole lines
+STAGE_STORES = os.environ.get("STENPY_STAGE_STORES", "1") == "1"
 COMBINE_MULTI_MAX = 16        # descriptors per sp_reduce_combine_multi launch
 BAND_MAX_FRACTION = 0.08
 
@@ -966,6 +968,11 @@ class KernelGen:
             edge_mode = "band"
         self.edge_mode = self._pick_edge_mode(edge_mode or EDGE_MODE)
         self._choose_modes()
+        # outputs of >= 4 channels written through a per-wave LDS transpose (coalesced lines)
+        self.stage = [oi for oi, o in enumerate(group.outputs)
+                      if STAGE_STORES and self.use_lds and not batched and sched.tx % 64 == 0
+                      and info[o].ch >= 4 and out_dtypes.get(o) != torch.bfloat16]
+        self.stage_c = max([info[group.outputs[oi]].ch for oi in self.stage] + [0])
 
     _COSTLY = {"exp", "log", "sqrt", "sin", "cos", "tanh"}
 
@@ -1221,7 +1228,8 @@ class KernelGen:
         return [i for i, L_ in enumerate(self.src_layout) if self.use_lds and L_["lds"]]
 
     def lds_bytes(self) -> int:
-        return self.lds_array_bytes() + 4 * (self.block_threads() // 64) * 4 + 16
+        stage = (self.sched.threads // 64) * 64 * getattr(self, "stage_c", 0) * 4
+        return self.lds_array_bytes() + stage + 4 * (self.block_threads() // 64) * 4 + 16
 
     # -- per-voxel code ------------------------------------------------------------------------
     def _emit_variant(self, mode) -> Tuple[str, str]:
@@ -1229,10 +1237,14 @@ class KernelGen:
         pos = e.origin()
         body = []
         flag = getattr(self, "_flag", False)
+        staged = mode[0] == "lds"
         for oi, nid in enumerate([] if flag else self.group.outputs):
             vals = e.val(nid, pos)
             C = len(vals)
             bf = self.out_dtypes.get(nid) == torch.bfloat16
+            if staged and oi in self.stage:
+                body += [f"A.ov{oi}[{c}] = {v};" for c, v in enumerate(vals)] + ["A.vflag = 1;"]
+                continue
             for c, v in enumerate(vals):
                 st = "st_bf16" if bf else "st_f32"
                 cast = "(unsigned short*)" if bf else "(float*)"
@@ -1369,6 +1381,10 @@ class KernelGen:
         acc += [f"    float dflag{i};" for i in range(ndet)]
         acc += [f"    float sc{i};" for i in range(nd)]
         acc.append("    float detreg;")
+        for oi in self.stage:
+            acc.append(f"    float ov{oi}[{self.info[g.outputs[oi]].ch}];")
+        if self.stage:
+            acc.append("    int vflag;")
         acc.append("    int dummy_;")
         lines.append("struct Acc {\n" + "\n".join(acc) + "\n};\n")
         off = 0
@@ -1446,6 +1462,8 @@ class KernelGen:
             kname = "fused_flag" if getattr(self, "_flag", False) else "fused_kernel"
             A(f'extern "C" __global__ void __launch_bounds__(NTB) {kname}(const Params p) {{')
             A("    __shared__ float red_scratch[4 * NTB / 64 + 4];")
+            if self.stage and not getattr(self, "_flag", False):
+                A(f"    __shared__ float so_stage[NT / 64][64 * {self.stage_c}];")
             A("    const int tid = threadIdx.x, rb = blockIdx.x;")
             # XCD-aware block remap: XCD k (hardware block id % 8) walks a contiguous tile range
             A("    const int b = blockIdx.x, nb = p.nblocks, per = nb >> 3, rem = nb & 7;")
@@ -1640,6 +1658,9 @@ class KernelGen:
         A(f"{ind}  #pragma unroll")
         A(f"{ind}  for (int k = 0; k < VPT; ++k) {{")
         A(f"{ind}    const int ly = ly0 + k * RP, y = y0 + ly, x = x0 + lx;")
+        staged = bool(self.stage) and not getattr(self, "_flag", False)
+        if staged:
+            A(f"{ind}    A.vflag = 0;")
         A(f"{ind}    if (y < N1 && x < N2) {{")
         A(f"{ind}      const IDX vox = ((IDX){z} * N1 + y) * N2 + x;")
         qa = "".join(f", qm{i}[k], q0{i}[k], {buf}o{i}[k]" for i in self.queue_sources())
@@ -1659,6 +1680,27 @@ class KernelGen:
         else:
             A(f"{ind}      {self._variant_name(('gather',))}{call}")
         A(f"{ind}    }}")
+        if staged:
+            # the wave's 64 voxels are consecutive in x (TX % 64 == 0): lane l deposits its C
+            # channels, then element q * 64 + l of the wave's C * 64 outputs is stored by lane l
+            A(f"{ind}    {{ const unsigned long long vm = __ballot(A.vflag);")
+            A(f"{ind}      if (vm) {{")
+            A(f"{ind}        float* st = so_stage[tid >> 6];")
+            A(f"{ind}        const int lane = tid & 63;")
+            A(f"{ind}        const IDX base = ((IDX){z} * N1 + y) * N2 + (x - lane);")
+            for oi in self.stage:
+                C = self.info[self.group.outputs[oi]].ch
+                A(f"{ind}        #pragma unroll")
+                A(f"{ind}        for (int c = 0; c < {C}; ++c) st[lane * {C} + c] = A.ov{oi}[c];")
+                A(f"{ind}        wave_sync();")
+                A(f"{ind}        #pragma unroll")
+                A(f"{ind}        for (int q = 0; q < {C}; ++q) {{")
+                A(f"{ind}          const int e = q * 64 + lane;")
+                A(f"{ind}          if ((vm >> (e / {C})) & 1ull) st_f32((float*)p.out{oi}, base * {C} + e, st[e]);")
+                A(f"{ind}        }}")
+                A(f"{ind}        wave_sync();")
+            A(f"{ind}      }}")
+            A(f"{ind}    }}")
         A(f"{ind}  }}")
         if self.fw:
             A(f"{ind}  }} else if (fact) {{")

@@ -113,3 +113,20 @@ def test_load_transforms_detected_only_where_they_pay():
         gen = comp[0].gen
         got = [g.node(n).op_name for n in gen.xform[0][0]] if gen.xform else None
         assert got == chain, (expr, got)
+
+
+@pytest.mark.parametrize("sched,staged", [((128, 8, 64, 256), True), ((64, 4, 64, 256), True),
+                                          ((32, 8, 64, 128), False)])
+def test_wide_outputs_staged_through_lds_only_on_64_wide_tiles(sched, staged):
+    # 9-channel hessian and 3-channel gradient: only outputs of >= 4 channels are staged
+    f = torch.zeros((16, 24, 200))
+    g, hes, *_ = sp.compile_expression("hessian(f)", boundary="periodic", field_map={"f": f})
+    grad = g.add("gradient", (g.node(hes).input_ids[0],), dict(g.node(hes).params))
+    values = {n: g.node(n).params["value"] for n in g._order if g.node(n).op_name == "_constant"}
+    steps, info = F.plan_graph(g, values, [hes, grad])
+    gen = F.KernelGen(g, steps[0].group, info, F.Schedule(*sched), {}, {})
+    outs = steps[0].group.outputs
+    assert gen.stage == ([outs.index(hes)] if staged else [])
+    src = gen.source()
+    assert ("so_stage" in src) == staged
+    F.compile_source(src)

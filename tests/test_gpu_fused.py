@@ -86,6 +86,19 @@ def test_hessian_tensor_ops(edge_mode):
         run_and_check(expr, {"f": f, "u": vfield(shape)}, boundary="periodic", edge_mode=edge_mode)
 
 
+@pytest.mark.parametrize("edge_mode", ["band", "variants"])
+@pytest.mark.parametrize("sched", [Schedule(128, 8, 16, 256), Schedule(64, 4, 16, 256)])
+def test_wide_outputs_staged_stores(sched, edge_mode):
+    # 9-channel outputs on 64-wide tiles go through the per-wave LDS transpose; x = 200 leaves
+    # a partial tile whose lanes past the edge store nothing
+    shape = (12, 20, 200)
+    f = field(shape, 8)
+    for bc in ("periodic", "neumann"):
+        for expr in ["hessian(f)", "deviatoric(hessian(f)) * 2", "jacobian(curl(u))"]:
+            run_and_check(expr, {"f": f, "u": vfield(shape)}, boundary=bc, schedule=sched,
+                          edge_mode=edge_mode)
+
+
 def test_scalar_from_reduction_feeds_next_group():
     shape = (16, 18, 66)
     f = field(shape, 31)
