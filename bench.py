@@ -173,13 +173,9 @@ def bench_tensor(args) -> int:
     del q
     exprs = ["eigenvalues(hessian(f))", "determinant(hessian(f))", "trace(hessian(f))",
              "deviatoric(hessian(f))", "mean_curvature(f)"]
-    from stenpy_amd.compiler import compile_expression
+    from stenpy_amd.compiler import compile_expressions
     # one graph, five sinks sharing the hessian: f is read once per evaluation
-    g, eig, *_ = compile_expression(exprs[0], boundary="periodic", field_map={"f": f})
-    hes = g.node(eig).input_ids[0]
-    src = g.node(hes).input_ids[0]
-    sinks = [eig] + [g.add(op, (hes,), {}) for op in ("determinant", "trace", "deviatoric")]
-    sinks.append(g.add("mean_curvature", (src,), dict(g.node(hes).params, boundary="periodic")))
+    g, sinks, *_ = compile_expressions(exprs, boundary="periodic", field_map={"f": f})
     graphs = [(g, sinks)]
     rt = sp.Runtime(device=dev, fusion_enabled=True, schedule_fn=tune.schedule_fn(dev))
 

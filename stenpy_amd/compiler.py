@@ -24,7 +24,7 @@ from __future__ import annotations
 
 import re
 from dataclasses import dataclass, field as dc_field
-from typing import Any, Dict, List, Optional, Set, Tuple
+from typing import Any, Dict, List, Optional, Sequence, Set, Tuple
 
 import sympy as sp
 import torch
@@ -294,3 +294,23 @@ def compile_expression(expr_str: str, dx: Any = 1.0, boundary: str = "neumann",
         keep = [sink] + list(ctx.field_map.values())
         g = g.dead_code_eliminate(keep)
     return g, sink, ctx.warnings, dict(ctx.field_map), sym
+
+
+def compile_expressions(exprs: Sequence[str], dx: Any = 1.0, boundary: str = "neumann",
+                        field_map: Optional[Dict[str, Any]] = None
+                        ) -> Tuple[Graph, List[str], List[str], Dict[str, str]]:
+    """Compile several expressions over the same fields into ONE graph with one sink each,
+    sharing common subexpressions across them (``hessian(f)`` under both
+    ``eigenvalues(hessian(f))`` and ``trace(hessian(f))`` is one node), so a fused run reads
+    each field once for all of them.  Returns ``(graph, sink_ids, warnings, field ids)``.
+    The reference compiles one expression per graph (stenpy_engine.py:3762-3778)."""
+    if field_map is None:
+        field_map = {"f": torch.zeros(1, dtype=torch.float64)}
+    g = Graph()
+    ctx = _CompileCtx(graph=g, dx=dx, boundary=boundary)
+    for name, value in field_map.items():
+        ctx.field_map[name] = g.add("_constant", (), {"value": value})
+    ctx.src_id = ctx.field_map.get("f", next(iter(ctx.field_map.values())))
+    sinks = [_lower(parse_expression(e, field_names=set(field_map)), ctx) for e in exprs]
+    g = g.dead_code_eliminate(sinks + list(ctx.field_map.values()))
+    return g, sinks, ctx.warnings, dict(ctx.field_map)

@@ -9,8 +9,6 @@
 //   inverse      adjugate / det                               (2724-2730)
 #pragma once
 
-#define SP_SYM3_FP64_BAND 0.05f  // 1-|r| below this: near-double-degenerate -> FP64 branch
-
 __device__ __forceinline__ float sym3_trace(const float* a) { return (a[0] + a[4]) + a[8]; }
 
 // a*d - b*c with one rounding error (Kahan's FMA scheme).
@@ -43,53 +41,68 @@ __device__ __forceinline__ void sort3(float* e) {
     if (e[0] > e[1]) { t = e[0]; e[0] = e[1]; e[1] = t; }
 }
 
-// Eigenvalues of a symmetric 3x3 (lower triangle, ascending, as eigvalsh), trigonometric closed
-// form: q = tr/3, p = sqrt(|A - qI|^2 / 6), r = det((A - qI)/p) / 2, eigenvalues
-// q + 2p cos(phi/3 + 2k pi/3) with phi = acos(r).  The angle is taken as
-// acos(r) = 2 atan2(sqrt(1 - r), sqrt(1 + r)), accurate where acos(r) itself is not (|r| -> 1).
-// Near a double eigenvalue (1 - |r| < SP_SYM3_FP64_BAND) r formed in FP32 has lost the digits
-// that matter, so there the moments q, p, r and the final combination are recomputed in FP64
-// (products of the FP32 entries); the trigonometry is FP32 in both branches.  A diagonal
-// matrix returns its sorted diagonal exactly.
-template <typename T>
-__device__ __forceinline__ void sym3_moments(const float* m, T& q, T& p, T& r) {
-    const T a00 = m[0], a11 = m[4], a22 = m[8];
-    const T a10 = m[3], a20 = m[6], a21 = m[7];
-    q = (a00 + a11 + a22) / T(3);
-    const T b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
-    const T p1 = a10 * a10 + a20 * a20 + a21 * a21;
-    const T p2 = b00 * b00 + b11 * b11 + b22 * b22 + T(2) * p1;
-    p = sqrt(p2 / T(6));
-    const T detB = b00 * (b11 * b22 - a21 * a21) - a10 * (a10 * b22 - a21 * a20) +
-                   a20 * (a10 * a21 - b11 * a20);
-    r = p > T(0) ? detB / (T(2) * p * p * p) : T(0);
-    r = r < T(-1) ? T(-1) : (r > T(1) ? T(1) : r);
-}
-template <typename T>
-__device__ __forceinline__ void sym3_from_moments(T q, T p, T r, float* e) {
-    const float sm = sqrtf((float)(T(1) - r)), sp = sqrtf((float)(T(1) + r));
-    const float phi = 2.0f * atan2f(sm, sp) / 3.0f;
-    const T c2 = (T)cosf(phi), c0 = (T)cosf(phi + 2.0943951023931954923f);
-    const T e2 = q + T(2) * p * c2, e0 = q + T(2) * p * c0;
-    e[0] = (float)e0;
-    e[1] = (float)(T(3) * q - e0 - e2);
-    e[2] = (float)e2;
-}
+// Eigenvalues of a symmetric 3x3 (lower triangle, ascending, as eigvalsh), all in FP32.
+// Trigonometric closed form: with q = tr/3, B = A - qI, p = sqrt(tr(B^2) / 6) the eigenvalues
+// are q + 2p cos(phi/3 + 2k pi/3), where cos(phi) = det(B) / (2p^3).  The angle is not taken
+// from that quotient (acos loses every digit of a near-double eigenvalue) but as
+//     phi = atan2(sqrt(D), 3 sqrt(3) det(B)),   D = prod_{i<j} (l_i - l_j)^2,
+// using sin(phi)^2 = 1 - cos(phi)^2 = D / (4 (tr(B^2)/2)^3).  D is evaluated as a sum of
+// squares, which has no cancellation: D = det(V^T V) for the Vandermonde V_ij = l_i^j, and
+// V^T V is the Gram matrix of I, B, B^2 under <X, Y> = tr(XY).  Writing a symmetric X as the
+// 6-vector (x00, x11, x22, sqrt2 x01, sqrt2 x02, sqrt2 x12), Cauchy-Binet turns that 3x3 Gram
+// determinant into the sum of the squared 3x3 minors of the 6x3 matrix [I | B | B^2]; the 19
+// nonzero minors group by how many diagonal rows they take (3, 2 or 1):
+//     D = m3^2 + 2 sum_{i<j, o} ((b_j - b_i) c_o - (c_j - c_i) b_o)^2 + 12 sum_{o<o'} (b_o c_o' - c_o b_o')^2
+// with b = B, c = B^2, i, j diagonal and o, o' off-diagonal positions.  Each minor carries an
+// absolute error of a few ulp of |B|^3, so the angle is accurate to a few ulp for every
+// spectrum, double and triple eigenvalues included.  B is scaled by its largest entry first,
+// which keeps |B|^6 inside the FP32 range.  A diagonal matrix returns its sorted diagonal exactly.
 __device__ __forceinline__ void sym3_eigvalsh(const float* m, float* e) {
     if (m[3] == 0.0f && m[6] == 0.0f && m[7] == 0.0f) {
         e[0] = m[0]; e[1] = m[4]; e[2] = m[8];
         sort3(e);
         return;
     }
-    float q, p, r;
-    sym3_moments<float>(m, q, p, r);
-    if (1.0f - fabsf(r) < SP_SYM3_FP64_BAND || !(r == r)) {
-        double qd, pd, rd;
-        sym3_moments<double>(m, qd, pd, rd);
-        sym3_from_moments<double>(qd, pd, rd, e);
-    } else {
-        sym3_from_moments<float>(q, p, r, e);
+    const float q = ((m[0] + m[4]) + m[8]) / 3.0f;
+    float b0 = m[0] - q, b1 = m[4] - q, b2 = m[8] - q, o01 = m[3], o02 = m[6], o12 = m[7];
+    const float mx = fmaxf(fmaxf(fmaxf(fabsf(b0), fabsf(b1)), fmaxf(fabsf(b2), fabsf(o01))),
+                           fmaxf(fabsf(o02), fabsf(o12)));
+    if (mx == 0.0f) {
+        // a multiple of I (fmaxf skips NaN: an off-diagonal NaN still reaches the result)
+        e[0] = e[1] = e[2] = q + 0.0f * ((o01 + o02) + o12);
+        return;
     }
+    const float s = 1.0f / mx;
+    b0 *= s; b1 *= s; b2 *= s; o01 *= s; o02 *= s; o12 *= s;
+    // C = B^2
+    const float c0 = b0 * b0 + o01 * o01 + o02 * o02;
+    const float c1 = o01 * o01 + b1 * b1 + o12 * o12;
+    const float c2 = o02 * o02 + o12 * o12 + b2 * b2;
+    const float c01 = b0 * o01 + o01 * b1 + o02 * o12;
+    const float c02 = b0 * o02 + o01 * o12 + o02 * b2;
+    const float c12 = o01 * o02 + b1 * o12 + o12 * b2;
+    // minors with the three diagonal rows, two, and one
+    const float m3 = ((b1 * c2 - b2 * c1) - (b0 * c2 - b2 * c0)) + (b0 * c1 - b1 * c0);
+    const float db[3] = {b1 - b0, b2 - b0, b2 - b1}, dc[3] = {c1 - c0, c2 - c0, c2 - c1};
+    const float bo[3] = {o01, o02, o12}, co[3] = {c01, c02, c12};
+    float s2 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            const float t = db[i] * co[o] - dc[i] * bo[o];
+            s2 = fmaf(t, t, s2);
+        }
+    const float t01 = o01 * c02 - c01 * o02, t02 = o01 * c12 - c01 * o12, t12 = o02 * c12 - c02 * o12;
+    const float s1 = (t01 * t01 + t02 * t02) + t12 * t12;
+    const float disc = fmaf(m3, m3, fmaf(2.0f, s2, 12.0f * s1));
+    const float detb = (b0 * (b1 * b2 - o12 * o12) - o01 * (o01 * b2 - o12 * o02)) +
+                       o02 * (o01 * o12 - b1 * o02);
+    const float phi = atan2f(sqrtf(disc), 5.19615242270663188f * detb) / 3.0f;
+    const float p2 = 2.0f * mx * sqrtf(((c0 + c1) + c2) / 6.0f);
+    e[2] = fmaf(p2, cosf(phi), q);
+    e[0] = fmaf(p2, cosf(phi + 2.0943951023931954923f), q);
+    e[1] = (3.0f * q - e[0]) - e[2];
     sort3(e);
 }
 

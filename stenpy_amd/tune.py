@@ -446,7 +446,8 @@ def main(argv=None) -> int:
     t.add_argument("--report", default=str(DATA_DIR / "schedule_mlp_report.json"))
     t.add_argument("--epochs", type=int, default=40)
     u = sub.add_parser("tune", help="exhaustive search for one expression; stores the winner")
-    u.add_argument("--expr", required=True)
+    u.add_argument("--expr", required=True, action="append",
+                   help="repeat to tune one graph with a sink per expression")
     u.add_argument("--shape", required=True, help="N0xN1xN2")
     u.add_argument("--boundary", default="periodic")
     u.add_argument("--reductions", default="")
@@ -483,14 +484,14 @@ def main(argv=None) -> int:
         return 0
     if a.cmd == "tune":
         from . import gen
-        from .compiler import compile_expression
+        from .compiler import compile_expressions
         shape = tuple(int(v) for v in a.shape.lower().split("x"))
         f = gen.generate(shape, 2, dev)
         u_ = gen.generate_vector(shape, (3, 4, 5), dev)
-        g, sink, *_ = compile_expression(a.expr, boundary=a.boundary, field_map={"f": f, "u": u_})
-        reds = [g.add(r, (sink,), {}) for r in a.reductions.split(",") if r]
+        g, sinks, *_ = compile_expressions(a.expr, boundary=a.boundary, field_map={"f": f, "u": u_})
+        reds = [g.add(r, (sinks[0],), {}) for r in a.reductions.split(",") if r]
         rt = Runtime(device=dev, fusion_enabled=True)
-        rows = autotune(g, rt, [sink] + reds)
+        rows = autotune(g, rt, sinks + reds)
         by = {}
         for r in rows:
             by.setdefault(r["sig"], []).append(r)

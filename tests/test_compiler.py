@@ -171,3 +171,17 @@ def test_keyword_arguments_need_a_call_without_nested_parentheses():
     assert node.op_name == "clamp" and node.params["lo"] == -1 and node.params["hi"] == 1
     with pytest.raises(ValueError):
         sp.compile_expression("clamp(laplacian(f), lo=-1, hi=1)", field_map={"f": f})
+
+
+def test_compile_expressions_shares_subexpressions_across_sinks():
+    from stenpy_amd.compiler import compile_expressions
+    f = torch.zeros((6, 7, 8))
+    exprs = ["eigenvalues(hessian(f))", "trace(hessian(f))", "mean_curvature(f)", "laplacian(f) + 1"]
+    g, sinks, _, fm = compile_expressions(exprs, boundary="periodic", field_map={"f": f})
+    ops = [g.node(n).op_name for n in g._order]
+    assert ops.count("hessian") == 1 and ops.count("_constant") == 2
+    assert [g.node(s).op_name for s in sinks] == ["eigenvalues", "trace", "mean_curvature", "add"]
+    # each sink is the same node structure the single-expression compiler builds
+    for e, s in zip(exprs, sinks):
+        g1, s1, *_ = sp.compile_expression(e, boundary="periodic", field_map={"f": f})
+        assert g1.node(s1).op_name == g.node(s).op_name and g1.node(s1).params == g.node(s).params

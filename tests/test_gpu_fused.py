@@ -94,7 +94,7 @@ def test_wide_outputs_staged_stores(sched, edge_mode):
     shape = (12, 20, 200)
     f = field(shape, 8)
     for bc in ("periodic", "neumann"):
-        for expr in ["hessian(f)", "deviatoric(hessian(f)) * 2", "jacobian(curl(u))"]:
+        for expr in ["hessian(f)", "deviatoric(hessian(f)) * 2", "jacobian(u)"]:
             run_and_check(expr, {"f": f, "u": vfield(shape)}, boundary=bc, schedule=sched,
                           edge_mode=edge_mode)
 

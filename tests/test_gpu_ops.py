@@ -186,21 +186,37 @@ def test_determinant_global_regularisation():
     assert_2x(ops.determinant(h), o64, o32, "det regularised")
 
 
-def test_eigenvalues_degenerate_branches():
+def test_eigenvalues_of_a_multiple_of_identity_are_exact():
     from stenpy_amd.gen import quadratic_host
     q = torch.from_numpy(quadratic_host((8, 9, 10))).cuda()
     e = ops.eigenvalues(ops.hessian(q, dx=1.0)).cpu()
     assert torch.equal(e, torch.full_like(e, 2.0)), "2I must give exactly (2, 2, 2)"
-    # near-double-degenerate matrices take the FP64 branch
+
+
+@pytest.mark.parametrize("spectrum", ["gap1e-1", "gap1e-4", "gap1e-7", "double", "triple",
+                                      "offset1e4", "tiny", "huge"])
+def test_eigenvalues_pernicious_spectra(spectrum):
+    # rotated diagonals: near-double, exactly double and near-triple eigenvalues, a large common
+    # offset, and magnitudes whose sixth powers leave the FP32 range
     g = torch.Generator().manual_seed(0)
-    Q, _ = torch.linalg.qr(torch.randn(4096, 3, 3, generator=g, dtype=torch.float64))
-    lam = torch.stack([torch.ones(4096, dtype=torch.float64),
-                       1 + 1e-4 * torch.randn(4096, generator=g, dtype=torch.float64),
-                       3 * torch.randn(4096, generator=g, dtype=torch.float64)], -1)
-    A = (Q @ torch.diag_embed(lam) @ Q.transpose(1, 2)).float()
+    n = 4096
+    Q, _ = torch.linalg.qr(torch.randn(n, 3, 3, generator=g, dtype=torch.float64))
+    one, r = torch.ones(n, dtype=torch.float64), lambda: torch.randn(n, generator=g, dtype=torch.float64)
+    scale = 1.0
+    if spectrum.startswith("gap"):
+        lam = [one, one + float(spectrum[3:]) * r(), 3 * r()]
+    elif spectrum == "double":
+        lam = [one, one, 3 * r()]
+    elif spectrum == "triple":
+        lam = [2 * one, 2 * one + 1e-6 * r(), 2 * one + 1e-6 * r()]
+    elif spectrum == "offset1e4":
+        lam = [1e4 * one, 1e4 * one + 1e-2 * r(), r()]
+    else:
+        lam, scale = [r(), r(), r()], (1e-25 if spectrum == "tiny" else 1e25)
+    A = (Q @ torch.diag_embed(torch.stack(lam, -1) * scale) @ Q.transpose(1, 2)).float()
     A = 0.5 * (A + A.transpose(1, 2))
     o64, o32 = ref(O.eigenvalues, A)
-    assert_2x(ops.eigenvalues(A.cuda()), o64, o32, "eig near-degenerate")
+    assert_2x(ops.eigenvalues(A.cuda()), o64, o32, f"eig {spectrum}")
 
 
 @pytest.mark.parametrize("bc", ["periodic", "neumann", "dirichlet", "reflect"])
