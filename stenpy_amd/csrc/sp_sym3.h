@@ -100,8 +100,11 @@ __device__ __forceinline__ void sym3_eigvalsh(const float* m, float* e) {
                        o02 * (o01 * o12 - b1 * o02);
     const float phi = atan2f(sqrtf(disc), 5.19615242270663188f * detb) / 3.0f;
     const float p2 = 2.0f * mx * sqrtf(((c0 + c1) + c2) / 6.0f);
-    e[2] = fmaf(p2, cosf(phi), q);
-    e[0] = fmaf(p2, cosf(phi + 2.0943951023931954923f), q);
+    float sn, cs;
+    sincosf(phi, &sn, &cs);
+    e[2] = fmaf(p2, cs, q);
+    // cos(phi + 2 pi / 3) = -cos(phi) / 2 - sin(phi) sqrt(3) / 2
+    e[0] = fmaf(p2, fmaf(-0.86602540378443865f, sn, -0.5f * cs), q);
     e[1] = (3.0f * q - e[0]) - e[2];
     sort3(e);
 }

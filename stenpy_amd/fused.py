@@ -1203,9 +1203,9 @@ class KernelGen:
     def _write_records(self, A, ndet: int, ind: str) -> None:
         flag = getattr(self, "_flag", False)
         if flag:
-            for i in range(ndet):
-                A(f"{ind}{{ RMinMax fm; fm.lo = INFINITY; fm.hi = A.dflag{i} > 0.0f ? 1.0f : -INFINITY;"
-                  f" block_mm(fm, red_scratch, p.dflag{i} + (i64)rb * 4); }}")
+            # any thread that saw a small diagonal sets the device flag (all write the same value)
+            cond = " || ".join(f"A.dflag{i} > 0.0f" for i in range(ndet))
+            A(f"{ind}if ({cond}) __hip_atomic_store((double*)p.detflag, 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);")
             return
         for ri, nid in enumerate(self.group.reductions):
             op = self.graph.node(nid).op_name
@@ -1434,6 +1434,9 @@ class KernelGen:
         A(f'extern "C" __global__ void __launch_bounds__(NTB) {kname}(const Params p) {{')
         A("    __shared__ float red_scratch[4 * NTB / 64 + 4];")
         A("    const int tid = threadIdx.x, nb = p.nblocks, b = nb + (int)blockIdx.x, rb = b;")
+        if getattr(self, "_flag", False):
+            # the flag is an "any": once one block has set it, blocks that start later stop
+            A("    if (__hip_atomic_load((double*)p.detflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0.0) return;")
         A("    Acc A;")
         for ri in range(len(g.reductions)):
             A(f"    A.r{ri}.init();")
@@ -1465,6 +1468,9 @@ class KernelGen:
             if self.stage and not getattr(self, "_flag", False):
                 A(f"    __shared__ float so_stage[NT / 64][64 * {self.stage_c}];")
             A("    const int tid = threadIdx.x, rb = blockIdx.x;")
+            if getattr(self, "_flag", False):
+                # the flag is an "any": once one block has set it, blocks that start later stop
+                A("    if (__hip_atomic_load((double*)p.detflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0.0) return;")
             # XCD-aware block remap: XCD k (hardware block id % 8) walks a contiguous tile range
             A("    const int b = blockIdx.x, nb = p.nblocks, per = nb >> 3, rem = nb & 7;")
         A("    Acc A;")
@@ -1982,10 +1988,9 @@ class FusedExecutor:
         stream = L.stream_ptr(dev)
         nb = ln.nblocks + cg.gen.boundary_blocks()
         if ln.dflags:
-            # flag pass, then the device-side any-small-diagonal flag the main pass reads
+            # flag pass: sets the device-side any-small-diagonal flag the main pass reads
+            ln.detflag.zero_()
             ln.launch(stream, entry="fused_flag")
-            L.check(L.lib().sp_reduce_combine(L.REDUCE["any_small_diag"], ln.dflags[0].data_ptr(), nb,
-                                              n0 * n1 * n2, ln.detflag.data_ptr(), stream), "det flag")
         ln.launch(stream)
         for o, t in zip(g.outputs, ln.outs):
             results[o] = t
