@@ -111,7 +111,11 @@ EDGE_MODE = os.environ.get("STENPY_EDGE_MODE", "auto")
 # The gather evaluation of the face band (z faces, y faces, corners, partial tiles) runs as a
 # separate small kernel (no LDS arrays, few registers) on a side stream, beside the LDS
 # z-marching kernel; off: the same work as extra blocks at the end of the main grid.
-SPLIT_TAIL = os.environ.get("STENPY_SPLIT_TAIL", "1") == "1"
+# where the edge band's gather blocks run: "auto" puts them at the end of the main grid unless
+# that code pushes the main kernel past MERGED_TAIL_MAX_VGPRS (below 4 waves per SIMD), in which
+# case they run as a separate kernel on a side stream; "1" / "0" force either
+SPLIT_TAIL = os.environ.get("STENPY_SPLIT_TAIL", "auto")
+MERGED_TAIL_MAX_VGPRS = 128
 # stage wide channel-last outputs through LDS so every store instruction writes whole lines
 STAGE_STORES = os.environ.get("STENPY_STAGE_STORES", "1") == "1"
 COMBINE_MULTI_MAX = 16        # descriptors per sp_reduce_combine_multi launch
@@ -1161,7 +1165,10 @@ class KernelGen:
 
     @property
     def split_tail(self) -> bool:
-        return SPLIT_TAIL and self.use_lds and not self.batched and self.boundary_blocks() > 0
+        if not (self.use_lds and not self.batched and self.boundary_blocks() > 0):
+            return False
+        want = getattr(self, "want_split", None)
+        return want if want is not None else SPLIT_TAIL == "1"
 
     def _boundary_part(self, A, ndet: int) -> None:
         """Tail blocks (block index b >= p.nblocks): gather evaluation of the boundary boxes."""
@@ -1778,6 +1785,31 @@ def compile_source(src: str, kernel: str = "fused_kernel") -> Tuple[bytes, float
     return code, dt
 
 
+def kernel_registers(code: bytes) -> Dict[str, int]:
+    """Vector registers (VGPRs + AGPRs) of each kernel of a gfx950 code object, read from its
+    AMDGPU metadata note (an ELF note of type 32 holding msgpack), without a GPU."""
+    import msgpack
+    shoff, = struct.unpack_from("<Q", code, 0x28)
+    shentsize, shnum = struct.unpack_from("<HH", code, 0x3A)
+    out: Dict[str, int] = {}
+    for i in range(shnum):
+        base = shoff + i * shentsize
+        sh_type, = struct.unpack_from("<I", code, base + 4)
+        off, size = struct.unpack_from("<QQ", code, base + 0x18)
+        if sh_type != 7:                                   # SHT_NOTE
+            continue
+        q = off
+        while q + 12 <= off + size:
+            nsz, dsz, typ = struct.unpack_from("<III", code, q)
+            d0 = q + 12 + ((nsz + 3) & ~3)
+            if typ == 32 and code[q + 12:q + 12 + nsz].rstrip(b"\0") == b"AMDGPU":
+                md = msgpack.unpackb(code[d0:d0 + dsz], raw=False)
+                for k in md.get("amdhsa.kernels", []):
+                    out[k[".name"]] = int(k.get(".vgpr_count", 0)) + int(k.get(".agpr_count", 0))
+            q = d0 + ((dsz + 3) & ~3)
+    return out
+
+
 def load_module(src: str, kernel: str = "fused_kernel") -> L.Module:
     h = hashlib.sha1(src.encode()).hexdigest()
     with _cache_lock:
@@ -1892,6 +1924,13 @@ class FusedExecutor:
             if gen.lds_bytes() > LDS_LIMIT:
                 raise RuntimeError(f"fused group needs {gen.lds_bytes()} B of LDS (limit {LDS_LIMIT})")
             src = gen.source()
+            if SPLIT_TAIL == "auto" and gen.use_lds and not gen.batched and gen.boundary_blocks() > 0:
+                # in-grid tail blocks beat the side-stream kernel unless their code costs the
+                # main kernel occupancy
+                code, _ = compile_source(src)
+                if kernel_registers(code).get("fused_kernel", 0) > MERGED_TAIL_MAX_VGPRS:
+                    gen.want_split = True
+                    src = gen.source()
             compiled[si] = CompiledGroup(gen, src)
         plan = (steps, info, compiled)
         self._plans[key] = plan

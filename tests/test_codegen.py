@@ -130,3 +130,19 @@ def test_wide_outputs_staged_through_lds_only_on_64_wide_tiles(sched, staged):
     src = gen.source()
     assert ("so_stage" in src) == staged
     F.compile_source(src)
+
+
+@pytest.mark.parametrize("expr", ["laplacian(f) + tanh(f)",
+                                  "eigenvalues(hessian(f)) * 1.5 + mean_curvature(f) + determinant(hessian(f))"])
+def test_tail_blocks_join_the_main_grid_unless_registers_grow(expr):
+    # the plan keeps the edge band's blocks in the main launch unless that code pushes the main
+    # kernel past the register limit (read from the code object's metadata, no GPU needed)
+    g, sink, reds, steps, comp = plan(expr, bc="periodic", shape=(40, 48, 96))
+    gen = comp[0].gen
+    merged = F.KernelGen(g, steps[0].group, gen.info, gen.sched, gen.out_dtypes, {}, gen.edge_mode)
+    merged.want_split = False
+    regs = F.kernel_registers(F.compile_source(merged.source())[0])
+    assert regs["fused_kernel"] > 0 and "fused_tail" not in regs
+    assert gen.split_tail == (regs["fused_kernel"] > F.MERGED_TAIL_MAX_VGPRS)
+    if gen.split_tail:
+        assert F.kernel_registers(F.compile_source(comp[0].source)[0])["fused_tail"] > 0
