@@ -256,7 +256,7 @@ def offline_reports(args) -> int:
     if args.pmc_traffic:
         t = profiling.traffic(args.pmc_traffic[0], args.pmc_traffic[1], kernel="fused_kernel",
                               copy_kernel="k_unary_contig", copy_bytes=copy_bytes(args.n))
-        try:        # the face-band tail kernel of the same group (side stream)
+        try:        # the edge-band tail kernel of the same group, when it runs separately
             tt = profiling.traffic(args.pmc_traffic[0], args.pmc_traffic[1], kernel="fused_tail",
                                    copy_kernel="k_unary_contig", copy_bytes=copy_bytes(args.n))
             t["tail_bytes"] = tt["bytes_per_launch"]
@@ -377,7 +377,7 @@ def main() -> int:
                    "speedup_vs_plain_both_eager_launch": round(t_plain / t_fused_eager, 2),
                    "bf16_out_gvoxel_s": round(nvox / t_bf16 / 1e9, 2),
                    "schedule_source": schedule_source(cg, g),
-                   "step": "hipGraph replay of the fused evaluation (main + tail kernels, 4 FP64 combines)",
+                   "step": "hipGraph replay of the fused evaluation (fused kernels, 4 FP64 combines)",
                    "eager_launch_gvoxel_s": round(nvox / t_fused_eager / 1e9, 2),
                    "eager_launch_ms_per_step": round(t_fused_eager * 1e3, 4),
                    "device_copy_gbs": copy_gbs,
@@ -385,7 +385,9 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel_ms": round(k_ms, 4),
-                     "kernels": "fused_kernel (LDS z-march, face wave) + fused_tail (face band, side stream)",
+                     "kernels": ("fused_kernel (LDS z-march, face wave) + fused_tail (edge band, side stream)"
+                                 if cg.gen.split_tail else
+                                 "fused_kernel (LDS z-march, face wave, edge-band blocks at the end of the grid)"),
                      "algorithmic_bytes_per_launch": alg_bytes},
     }
     if not args.no_cpu:
