@@ -373,7 +373,7 @@ def evaluate_choice(rows: List[Dict[str, Any]], model: ScheduleModel, k: int = 1
     by: Dict[Tuple, List[Dict[str, Any]]] = {}
     for r in rows:
         by.setdefault((r.get("graph"), r["sig"]), []).append(r)
-    regrets, gains, hits = [], [], 0
+    regrets, gains, ceiling, hits = [], [], [], 0
     for key, rs in by.items():
         pred = np.asarray(model.predict_rows(rs)).reshape(-1)
         pick = min(rs[i]["ms"] for i in np.argsort(pred)[:k])
@@ -382,9 +382,13 @@ def evaluate_choice(rows: List[Dict[str, Any]], model: ScheduleModel, k: int = 1
         regrets.append(pick / best - 1.0)
         if dflt is not None:
             gains.append(dflt / pick)
+            ceiling.append(dflt / best)
         hits += pick == best
+    geo = lambda v: float(np.exp(np.mean(np.log(v)))) if v else None
+    # ceiling: the exhaustive search's own gain over the heuristic, the most any pick can reach
     return {"k": k, "groups": len(by), "mean_regret": float(np.mean(regrets)), "max_regret": float(np.max(regrets)),
-            "top1": hits / max(len(by), 1), "geomean_gain_vs_default": float(np.exp(np.mean(np.log(gains)))) if gains else None}
+            "top1": hits / max(len(by), 1), "geomean_gain_vs_default": geo(gains),
+            "geomean_ceiling_gain_vs_default": geo(ceiling)}
 
 
 # ------------------------------------------------------------------------------ command line

@@ -49,10 +49,13 @@ def test_learned_schedules_beat_the_heuristic_on_heldout_graphs():
     assert held["groups"] >= 5
     assert rep["loss_last"] < 0.5 * rep["loss_first"]
     # the network's pick is on average faster than the heuristic's and close to exhaustive;
-    # timing its three best proposals comes closer still
-    assert held["geomean_gain_vs_default"] > 1.05, held
-    assert held["mean_regret"] < 0.2, held
-    assert top3["mean_regret"] < 0.08 and top3["mean_regret"] <= held["mean_regret"], top3
+    # timing its three best proposals comes closer still.  The gain is measured against the
+    # most any pick can gain over the heuristic (the exhaustive search's own gain), in log terms
+    share = lambda r: math.log(r["geomean_gain_vs_default"]) / math.log(r["geomean_ceiling_gain_vs_default"])
+    assert held["geomean_gain_vs_default"] > 1.0 and share(held) > 0.4, held
+    assert share(top3) > 0.7, top3
+    assert held["mean_regret"] < 0.1, held
+    assert top3["mean_regret"] < 0.05 and top3["mean_regret"] <= held["mean_regret"], top3
 
 
 @pytest.mark.skipif(not tune.MODEL_PATH.exists(), reason="no committed model")
