@@ -976,7 +976,12 @@ class KernelGen:
         self.stage = [oi for oi, o in enumerate(group.outputs)
                       if STAGE_STORES and self.use_lds and not batched and sched.tx % 64 == 0
                       and info[o].ch >= 4 and out_dtypes.get(o) != torch.bfloat16]
-        self.stage_c = max([info[group.outputs[oi]].ch for oi in self.stage] + [0])
+        # per wave: 64 * C floats per staged output, deposited by the lanes inside the variant
+        self.stage_off: Dict[int, int] = {}
+        self.stage_c = 0
+        for oi in self.stage:
+            self.stage_off[oi] = 64 * self.stage_c
+            self.stage_c += info[group.outputs[oi]].ch
 
     _COSTLY = {"exp", "log", "sqrt", "sin", "cos", "tanh"}
 
@@ -1250,7 +1255,11 @@ class KernelGen:
             C = len(vals)
             bf = self.out_dtypes.get(nid) == torch.bfloat16
             if staged and oi in self.stage:
-                body += [f"A.ov{oi}[{c}] = {v};" for c, v in enumerate(vals)] + ["A.vflag = 1;"]
+                # deposit into this lane's slot of the wave's staging area; the wave stores the
+                # whole area with coalesced lines after the variant
+                body.append(f"{{ float* st_ = &so_stage[threadIdx.x >> 6][{self.stage_off[oi]} + (threadIdx.x & 63) * {C}];")
+                body += [f"  st_[{c}] = {v};" for c, v in enumerate(vals)]
+                body += ["}", "A.vflag = 1;"]
                 continue
             for c, v in enumerate(vals):
                 st = "st_bf16" if bf else "st_f32"
@@ -1388,8 +1397,6 @@ class KernelGen:
         acc += [f"    float dflag{i};" for i in range(ndet)]
         acc += [f"    float sc{i};" for i in range(nd)]
         acc.append("    float detreg;")
-        for oi in self.stage:
-            acc.append(f"    float ov{oi}[{self.info[g.outputs[oi]].ch}];")
         if self.stage:
             acc.append("    int vflag;")
         acc.append("    int dummy_;")
@@ -1405,6 +1412,8 @@ class KernelGen:
                 # one spare row per channel (the last channel's absorbs the stores of unused
                 # halo entries)
                 lines.append(f"__shared__ float lds_s{i}{dims};")
+        if self.stage:
+            lines.append(f"__shared__ float so_stage[NT / 64][{64 * self.stage_c}];")
         lines += self._xform_functions()
         fns = []
         for m in self._variants():
@@ -1472,8 +1481,6 @@ class KernelGen:
             kname = "fused_flag" if getattr(self, "_flag", False) else "fused_kernel"
             A(f'extern "C" __global__ void __launch_bounds__(NTB) {kname}(const Params p) {{')
             A("    __shared__ float red_scratch[4 * NTB / 64 + 4];")
-            if self.stage and not getattr(self, "_flag", False):
-                A(f"    __shared__ float so_stage[NT / 64][64 * {self.stage_c}];")
             A("    const int tid = threadIdx.x, rb = blockIdx.x;")
             if getattr(self, "_flag", False):
                 # the flag is an "any": once one block has set it, blocks that start later stop
@@ -1694,24 +1701,23 @@ class KernelGen:
             A(f"{ind}      {self._variant_name(('gather',))}{call}")
         A(f"{ind}    }}")
         if staged:
-            # the wave's 64 voxels are consecutive in x (TX % 64 == 0): lane l deposits its C
-            # channels, then element q * 64 + l of the wave's C * 64 outputs is stored by lane l
+            # the wave's 64 voxels are consecutive in x (TX % 64 == 0): each lane deposited its C
+            # channels inside the variant; element q * 64 + l of the wave's C * 64 outputs is
+            # stored by lane l
             A(f"{ind}    {{ const unsigned long long vm = __ballot(A.vflag);")
             A(f"{ind}      if (vm) {{")
-            A(f"{ind}        float* st = so_stage[tid >> 6];")
             A(f"{ind}        const int lane = tid & 63;")
             A(f"{ind}        const IDX base = ((IDX){z} * N1 + y) * N2 + (x - lane);")
+            A(f"{ind}        wave_sync();")
             for oi in self.stage:
                 C = self.info[self.group.outputs[oi]].ch
-                A(f"{ind}        #pragma unroll")
-                A(f"{ind}        for (int c = 0; c < {C}; ++c) st[lane * {C} + c] = A.ov{oi}[c];")
-                A(f"{ind}        wave_sync();")
-                A(f"{ind}        #pragma unroll")
-                A(f"{ind}        for (int q = 0; q < {C}; ++q) {{")
-                A(f"{ind}          const int e = q * 64 + lane;")
-                A(f"{ind}          if ((vm >> (e / {C})) & 1ull) st_f32((float*)p.out{oi}, base * {C} + e, st[e]);")
-                A(f"{ind}        }}")
-                A(f"{ind}        wave_sync();")
+                A(f"{ind}        {{ const float* st = &so_stage[tid >> 6][{self.stage_off[oi]}];")
+                A(f"{ind}          #pragma unroll")
+                A(f"{ind}          for (int q = 0; q < {C}; ++q) {{")
+                A(f"{ind}            const int e = q * 64 + lane;")
+                A(f"{ind}            if ((vm >> (e / {C})) & 1ull) st_f32((float*)p.out{oi}, base * {C} + e, st[e]);")
+                A(f"{ind}          }} }}")
+            A(f"{ind}        wave_sync();")
             A(f"{ind}      }}")
             A(f"{ind}    }}")
         A(f"{ind}  }}")
