@@ -1845,9 +1845,6 @@ class CompiledGroup:
             self.others[name] = L.Module(code, name)
         return self.others[name]
 
-    def tail_module(self) -> L.Module:
-        return self.kernel("fused_tail")
-
     @property
     def group(self) -> Group:
         return self.gen.group
