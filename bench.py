@@ -349,7 +349,9 @@ def main() -> int:
     t_plain = time_steps(plain, max(3, args.steps // 4), 1)
 
     cg, ln = group_launch(rt, g, res, sink)
-    k_ms = ln.time_ms(reps=max(10, args.steps))
+    # average duration of the group's kernels over the timed launches (HIP events on their
+    # stream), the same launch sequence the kernel-statistics profile records
+    k_ms = ln.time_ms(reps=args.steps, warmup=args.warmup, rounds=1)
     copy_gbs = device_copy_gbs(dev)
     alg_bytes = 20 * nvox
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
