@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -196,6 +197,37 @@ def bench_tensor(args) -> int:
     return 0
 
 
+def bench_spectral(args) -> int:
+    """512^3 FP32 field (seed 2): spectral Laplacian, Re(ifftn(-|k|^2 fftn(f))), with the
+    transforms in FP64 (rocFFT), against torch.fft on the same GPU in FP32 and FP64."""
+    import torch
+    from stenpy_amd import gen, ops
+    dev = torch.device("cuda:0")
+    n = args.n
+    f = gen.generate((n, n, n), 2, dev)
+    steps = max(3, args.steps // 4)
+    t = time_steps(lambda: ops.spectral_laplacian(f), steps, 1)
+    k = [torch.fft.fftfreq(n, d=1.0, device=dev) * (2 * math.pi) for _ in range(3)]
+    k2 = (k[0].view(-1, 1, 1) ** 2 + k[1].view(1, -1, 1) ** 2 + k[2].view(1, 1, -1) ** 2)
+
+    def torch_lap(x, kk):
+        return torch.fft.ifftn(-kk * torch.fft.fftn(x)).real
+    t32 = time_steps(lambda: torch_lap(f, k2), steps, 1)
+    f64, k64 = f.double(), k2.double()
+    t64 = time_steps(lambda: torch_lap(f64, k64), steps, 1)
+    del f64, k64
+    nvox = n ** 3
+    alg = 8 * nvox        # read f, write the result
+    _line("spectral Laplacian: Gvoxel/s on a 512^3 field (FP64 transforms, FP32 storage)",
+          round(nvox / t / 1e9, 3), "Gvoxel/s", steps, 1, t * 1e3,
+          {"workload": f"{n}^3 FP32 Fourier field seed 2, dx = 1",
+           "torch_fft_fp32_ms": round(t32 * 1e3, 3), "torch_fft_fp64_ms": round(t64 * 1e3, 3)},
+          roofline={"bound": "hbm", "achieved": round(alg / t / 1e9, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(alg / t / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                    "algorithmic_bytes_per_launch": alg})
+    return 0
+
+
 def bench_stream(args) -> int:
     """4096 x 4096 x 4608 field generated on the host slab by slab (256 planes, 1-plane halo),
     streamed through pinned buffers; a bounded sample of ``--slabs`` slabs is timed."""
@@ -296,11 +328,11 @@ def main() -> int:
                     help="render a rocprofv3 --stats CSV output as markdown (no GPU)")
     ap.add_argument("--cmd", type=str, default="", help="command line quoted in --stats-md output")
     ap.add_argument("--out", type=str, default=None)
-    ap.add_argument("--workload", choices=["graph", "batched", "stream", "tensor"], default="graph",
+    ap.add_argument("--workload", choices=["graph", "batched", "stream", "tensor", "spectral"], default="graph",
                     help="graph: the 512^3 stencil graph (default); batched: 96 uneven grids x 6 "
                          "random graphs in batched persistent launches; stream: out-of-core "
                          "4608x4096x4096 stream (bounded sample of slabs); tensor: 768^3 hessian "
-                         "tensor ops")
+                         "tensor ops; spectral: 512^3 spectral Laplacian")
     ap.add_argument("--slabs", type=int, default=2, help="stream: slabs of 256 planes to stream")
     args = ap.parse_args()
     if args.pmc_traffic or args.stats_md:
@@ -311,7 +343,8 @@ def main() -> int:
     if args.profile_pass:
         return profile_pass(args)
     if args.workload != "graph":
-        return {"batched": bench_batched, "stream": bench_stream, "tensor": bench_tensor}[args.workload](args)
+        return {"batched": bench_batched, "stream": bench_stream, "tensor": bench_tensor,
+                "spectral": bench_spectral}[args.workload](args)
 
     import torch
     import stenpy_amd as sp

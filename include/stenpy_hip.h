@@ -24,13 +24,13 @@
 extern "C" {
 #endif
 
-#define SP_ABI_VERSION 4
+#define SP_ABI_VERSION 5
 
 /* status codes */
 enum { SP_OK = 0, SP_EINVAL = 1, SP_EHIP = 2, SP_EUNSUPPORTED = 3, SP_ECOMPILE = 4 };
 
-/* element types */
-enum { SP_F32 = 0, SP_BF16 = 1, SP_F64 = 2 };
+/* element types; SP_C64 is interleaved (re, im) FP32 pairs, torch.complex64 */
+enum { SP_F32 = 0, SP_BF16 = 1, SP_F64 = 2, SP_C64 = 3 };
 
 /* boundary handling of padded stencils (stenpy_engine.py:1374-1391): neumann -> replicate,
  * dirichlet -> zeros, periodic -> circular, reflect -> mirror; any other name pads zeros. */
@@ -98,6 +98,23 @@ int sp_axis_pass(const sp_tensor* x, int axis, sp_tensor* out, int mode, int pad
 #define SP_FIR_MAX_TAPS 31
 int sp_axis_fir(const sp_tensor* x, int axis, sp_tensor* out, int ntaps, const double* weights,
                 int pad, void* stream);
+
+/* ---- spectral (stenpy_engine.py:2871-3012) -------------------------------------------- */
+/* torch.fft ``norm``: "backward" scales the inverse by 1/n, "forward" the forward transform,
+ * "ortho" both by 1/sqrt(n). */
+enum { SP_NORM_BACKWARD = 0, SP_NORM_ORTHO = 1, SP_NORM_FORWARD = 2 };
+/* FFT over every axis of a contiguous 1-D, 2-D or 3-D field, computed in FP64 (rocFFT) and
+ * rounded once.  x: FP32 (real) or C64.  Forward: out C64 -- fft / fft_complex, 2875-2885,
+ * 2901-2908.  Inverse: out C64 (ifft_complex, 2910-2918) or FP32, the real part (ifft,
+ * 2887-2896). */
+int sp_fft(const sp_tensor* x, sp_tensor* out, int inverse, int norm, void* stream);
+/* out = Re(ifftn(m(k) * fftn(x))) in FP64, rounded once, with k = 2 pi fftfreq(n_d, dx[d])
+ * (fftfreq in FP32, as torch's default dtype computes it) and m = i k_dim
+ * (SP_SPECTRAL_GRADIENT, spectral_gradient, 2979-2992) or -sum_d k_d^2 (SP_SPECTRAL_LAPLACIAN,
+ * spectral_laplacian, 2997-3010).  x, out: FP32, contiguous, 1-D to 3-D; dx: one spacing per
+ * axis. */
+enum { SP_SPECTRAL_GRADIENT = 0, SP_SPECTRAL_LAPLACIAN = 1 };
+int sp_spectral(const sp_tensor* x, sp_tensor* out, const double* dx, int kind, int dim, void* stream);
 
 /* 3-D single-pass kernels for scalar (n0,n1,n2) / vector (n0,n1,n2,3) fields.
  * Each replaces the CPU branch of the cited reference op (the parity target). */

@@ -450,6 +450,71 @@ def deviatoric(t: Tensor) -> Tensor:                              # 2852-2867
     return torch.nan_to_num(r, nan=0.0, posinf=0.0, neginf=0.0)
 
 
+# ----------------------------------------------------------------------------- spectral
+# The reference works in FP64 / complex128; with FP32 input the same code is the FP32 twin
+# (complex64).  Wave numbers: torch.fft.fftfreq in torch's default dtype (FP32), widened to the
+# working dtype and scaled by 2 pi (stenpy_engine.py:2931-2964).
+def _complex_of(t: Tensor) -> torch.dtype:
+    if torch.is_complex(t):
+        return t.dtype
+    return torch.complex128 if t.dtype == torch.float64 else torch.complex64
+
+
+def _real_of(t: Tensor) -> torch.dtype:
+    return torch.float64 if t.dtype in (torch.float64, torch.complex128) else torch.float32
+
+
+def fft(t: Tensor, norm: str = "backward") -> Tensor:              # 2875-2885
+    if not torch.is_complex(t):
+        t = t.to(_complex_of(t))
+    return torch.fft.fftn(t, norm=norm)
+
+
+def ifft(t: Tensor, norm: str = "backward") -> Tensor:             # 2887-2896
+    if not torch.is_complex(t):
+        t = torch.complex(t, torch.zeros_like(t))
+    return torch.fft.ifftn(t, norm=norm).real.to(_real_of(t))
+
+
+def fft_complex(t: Tensor, norm: str = "backward") -> Tensor:      # 2901-2908
+    if norm == "numpy_compatible":
+        norm = "backward"
+    return torch.fft.fftn(t, norm=norm)
+
+
+def ifft_complex(t: Tensor, norm: str = "backward") -> Tensor:     # 2910-2918
+    if not torch.is_complex(t):
+        t = torch.complex(t, torch.zeros_like(t))
+    if norm == "numpy_compatible":
+        norm = "backward"
+    return torch.fft.ifftn(t, norm=norm)
+
+
+def _k_grids(shape, spacing, dtype):                                # 2931-2964
+    grids = []
+    for d, (n, dx) in enumerate(zip(shape, spacing)):
+        k = torch.fft.fftfreq(n, d=dx).to(dtype) * (2 * math.pi)
+        sh = [1] * len(shape)
+        sh[d] = -1
+        grids.append(k.view(sh))
+    return grids, sum(kg ** 2 for kg in grids)
+
+
+def spectral_gradient(t: Tensor, dx: Any = 1.0, dim: int = 0, norm: str = "backward") -> Tensor:
+    # 2979-2992
+    dxs = (float(dx),) * t.ndim if isinstance(dx, (int, float)) else tuple(dx)
+    grids, _ = _k_grids(t.shape, dxs, t.dtype)
+    T_hat = torch.fft.fftn(t, norm=norm)
+    return torch.fft.ifftn(1j * grids[dim] * T_hat, norm=norm).real.to(t.dtype)
+
+
+def spectral_laplacian(t: Tensor, dx: Any = 1.0, norm: str = "backward") -> Tensor:   # 2997-3010
+    dxs = (float(dx),) * t.ndim if isinstance(dx, (int, float)) else tuple(dx)
+    _, k2 = _k_grids(t.shape, dxs, t.dtype)
+    T_hat = torch.fft.fftn(t, norm=norm)
+    return torch.fft.ifftn(-k2 * T_hat, norm=norm).real.to(t.dtype)
+
+
 OPS = {
     "add": add, "sub": sub, "mul": mul, "div": div, "neg": neg, "clamp": clamp, "exp": exp,
     "log": log, "sqrt": sqrt, "sin": sin, "cos": cos, "tanh": tanh,
@@ -466,4 +531,6 @@ OPS = {
     "cumulative_integral": cumulative_integral,
     "trace": trace, "determinant": determinant, "eigenvalues": eigenvalues,
     "inverse": inverse, "deviatoric": deviatoric,
+    "fft": fft, "ifft": ifft, "fft_complex": fft_complex, "ifft_complex": ifft_complex,
+    "spectral_gradient": spectral_gradient, "spectral_laplacian": spectral_laplacian,
 }

@@ -16,10 +16,10 @@ import numpy as np
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "libstenpy_hip.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # enums mirrored from include/stenpy_hip.h
-SP_F32, SP_BF16, SP_F64 = 0, 1, 2
+SP_F32, SP_BF16, SP_F64, SP_C64 = 0, 1, 2, 3
 BOUNDARY = {"periodic": 0, "neumann": 1, "dirichlet": 2, "reflect": 3}
 SP_ZEROPAD = 4
 UNARY = {"neg": 0, "exp": 1, "log": 2, "sqrt": 3, "sin": 4, "cos": 5, "tanh": 6, "clamp": 7,
@@ -30,7 +30,8 @@ REDUCE = {"sum": 0, "mean": 1, "norm_l2": 2, "variance": 3, "variance_biased": 4
           "min_max": 5, "integrate": 6, "any_small_diag": 7}
 TENSOR3 = {"trace": 0, "determinant": 1, "eigenvalues": 2, "deviatoric": 3, "inverse": 4}
 
-_DTYPE_CODE = {torch.float32: SP_F32, torch.bfloat16: SP_BF16, torch.float64: SP_F64}
+_DTYPE_CODE = {torch.float32: SP_F32, torch.bfloat16: SP_BF16, torch.float64: SP_F64,
+               torch.complex64: SP_C64}
 
 
 def boundary_code(name: str) -> int:
@@ -74,6 +75,8 @@ def _declare(lib: C.CDLL) -> None:
         "sp_op_norm_last": ([P, P, D, V], I),
         "sp_op_normalize_last": ([P, P, D, V], I),
         "sp_axis_fir": ([P, I, P, I, C.POINTER(D), I, V], I),
+        "sp_fft": ([P, P, I, I, V], I),
+        "sp_spectral": ([P, P, C.POINTER(D), I, I, V], I),
         "sp_axis_pass": ([P, I, P, I, I, D, D, I, V], I),
         "sp_laplacian3d": ([P, P, C.POINTER(D), I, V], I),
         "sp_gradient3d": ([P, P, D, I, V], I),
@@ -210,6 +213,27 @@ def axis_fir(x: torch.Tensor, axis: int, out: torch.Tensor, weights, pad: int) -
     w = (C.c_double * len(weights))(*[float(v) for v in weights])
     check(lib().sp_axis_fir(C.byref(desc(x)), axis, C.byref(desc(out)), len(weights), w, pad,
                             stream_ptr(x.device)), "axis fir")
+    return out
+
+
+NORM = {"backward": 0, "ortho": 1, "forward": 2}
+SPECTRAL_GRADIENT, SPECTRAL_LAPLACIAN = 0, 1
+
+
+def fft(x: torch.Tensor, out: torch.Tensor, inverse: bool, norm: str) -> torch.Tensor:
+    require_gpu(x, out)
+    if norm not in NORM:
+        raise ValueError(f"unknown FFT norm {norm!r} (backward, ortho or forward)")
+    check(lib().sp_fft(C.byref(desc(x)), C.byref(desc(out)), int(bool(inverse)), NORM[norm],
+                       stream_ptr(x.device)), "fft")
+    return out
+
+
+def spectral(x: torch.Tensor, out: torch.Tensor, dx, kind: int, dim: int = 0) -> torch.Tensor:
+    require_gpu(x, out)
+    d = (C.c_double * x.ndim)(*[float(v) for v in dx])
+    check(lib().sp_spectral(C.byref(desc(x)), C.byref(desc(out)), d, int(kind), int(dim),
+                            stream_ptr(x.device)), "spectral")
     return out
 
 

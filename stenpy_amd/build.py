@@ -67,7 +67,7 @@ def build(verbose: bool = True) -> Path:
         objs = [f.result() for f in futs]
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-fopenmp", "-o", str(tmp),
-           *map(str, objs), "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"]
+           *map(str, objs), "-L/opt/rocm/lib", "-lhiprtc", "-lrocfft", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")

@@ -379,6 +379,77 @@ register_operator("conv3d_spectral", conv3d_spectral, radius=1, halo_l=1, halo_r
                   flops=2, bytes_in=8, bytes_out=8, kind="stencil")
 
 
+# ----------------------------------------------------------------------------- spectral
+def _c64_or_f32(t, dev) -> torch.Tensor:
+    if not isinstance(t, torch.Tensor):
+        t = torch.tensor(float(t))
+    if torch.is_complex(t):
+        t = t.to(torch.complex64)
+    elif t.dtype != F32:
+        t = t.to(F32)
+    if t.device != dev:
+        t = t.to(dev)
+    return _contig(t)
+
+
+def _fft_norm(norm: str) -> str:
+    return "backward" if norm == "numpy_compatible" else norm
+
+
+def fft(t, norm: str = "backward", alloc=None):
+    """n-D FFT over every axis, complex64 result computed in FP64 (stenpy_engine.py:2875-2885;
+    the reference returns complex128)."""
+    dev = _device_of(t)
+    t = _c64_or_f32(t, dev)
+    return L.fft(t, _empty(t.shape, dev, torch.complex64), False, _fft_norm(norm))
+
+
+def ifft(t, norm: str = "backward", alloc=None):
+    """Real part of the n-D inverse FFT, FP32 (stenpy_engine.py:2887-2896)."""
+    dev = _device_of(t)
+    t = _c64_or_f32(t, dev)
+    return L.fft(t, _empty(t.shape, dev), True, _fft_norm(norm))
+
+
+def fft_complex(t, norm: str = "backward", alloc=None):
+    """n-D FFT, complex64; ``norm="numpy_compatible"`` means "backward"
+    (stenpy_engine.py:2901-2908)."""
+    return fft(t, norm)
+
+
+def ifft_complex(t, norm: str = "backward", alloc=None):
+    """n-D inverse FFT keeping the complex result (stenpy_engine.py:2910-2918)."""
+    dev = _device_of(t)
+    t = _c64_or_f32(t, dev)
+    return L.fft(t, _empty(t.shape, dev, torch.complex64), True, _fft_norm(norm))
+
+
+def spectral_gradient(t, dx: Any = 1.0, dim: int = 0, norm: str = "backward", alloc=None):
+    """Derivative along ``dim`` through the FFT: Re(ifftn(i k_dim fftn(t))), k = 2 pi fftfreq
+    (stenpy_engine.py:2979-2992).  The transforms' norm cancels, as in the reference."""
+    dev = _device_of(t)
+    t = _contig(_f32(t, dev))
+    dim = int(dim)
+    if not -t.ndim <= dim < t.ndim:
+        raise IndexError(f"spectral_gradient: dim {dim} out of range for a {t.ndim}-D field")
+    return L.spectral(t, _empty(t.shape, dev), _dxs(dx, t.ndim), L.SPECTRAL_GRADIENT, dim % t.ndim)
+
+
+def spectral_laplacian(t, dx: Any = 1.0, norm: str = "backward", alloc=None):
+    """Laplacian through the FFT: Re(ifftn(-|k|^2 fftn(t))) (stenpy_engine.py:2997-3010)."""
+    dev = _device_of(t)
+    t = _contig(_f32(t, dev))
+    return L.spectral(t, _empty(t.shape, dev), _dxs(dx, t.ndim), L.SPECTRAL_LAPLACIAN)
+
+
+register_operator("fft", fft, cost="high")
+register_operator("ifft", ifft, cost="high")
+register_operator("fft_complex", fft_complex, cost="high")
+register_operator("ifft_complex", ifft_complex, cost="high")
+register_operator("spectral_gradient", spectral_gradient, cost="high")
+register_operator("spectral_laplacian", spectral_laplacian, cost="high")
+
+
 # ----------------------------------------------------------------------------- channel-last helpers
 def stack_components(*tensors, dim: int = -1):
     dev = _device_of(*tensors)

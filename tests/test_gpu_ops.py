@@ -12,9 +12,13 @@ SHAPE = (20, 24, 28)
 
 
 def ref(fn, *args, **kw):
-    """(oracle64, ref32) of ``fn`` on the FP32 device inputs."""
-    a64 = [a.detach().cpu().double() if isinstance(a, torch.Tensor) else a for a in args]
-    a32 = [a.detach().cpu().float() if isinstance(a, torch.Tensor) else a for a in args]
+    """(oracle64, ref32) of ``fn`` on the FP32 (or complex64) device inputs."""
+    def up(a, real, cplx):
+        if not isinstance(a, torch.Tensor):
+            return a
+        return a.detach().cpu().to(cplx if torch.is_complex(a) else real)
+    a64 = [up(a, torch.float64, torch.complex128) for a in args]
+    a32 = [up(a, torch.float32, torch.complex64) for a in args]
     return fn(*a64, **kw), fn(*a32, **kw)
 
 
@@ -243,3 +247,59 @@ def test_conv3d_spectral_and_expressions():
         rt = sp.Runtime(device="cuda")
         res = rt.run(g)[sink]
         assert_2x(res, graph_eval.evaluate(g, torch.float64)[sink], graph_eval.evaluate(g, torch.float32)[sink], expr)
+
+
+# ----------------------------------------------------------------------------- spectral
+def _ri(t):
+    """Complex results compared component-wise under the 2x rule."""
+    return torch.view_as_real(t.to(torch.complex128)) if torch.is_complex(t) else t
+
+
+@pytest.mark.parametrize("shape", [(64,), (12, 20), (10, 12, 15)])
+@pytest.mark.parametrize("norm", ["backward", "ortho", "forward"])
+def test_fft_family(shape, norm):
+    f = field(shape if len(shape) == 3 else (1, 1) + shape if len(shape) == 1 else (1,) + shape, 41)
+    f = f.reshape(shape).contiguous()
+    F_ = ops.fft(f, norm=norm)
+    assert F_.dtype == torch.complex64 and F_.shape == f.shape
+    o64, o32 = ref(O.fft, f, norm=norm)
+    assert_2x(_ri(F_), _ri(o64), _ri(o32), f"fft {shape} {norm}")
+    for name in ("ifft", "ifft_complex"):
+        out = getattr(ops, name)(F_, norm=norm)
+        o64, o32 = ref(getattr(O, name), F_, norm=norm)
+        assert_2x(_ri(out), _ri(o64), _ri(o32), f"{name} {shape} {norm}")
+    # a real input to the inverse is a complex one with zero imaginary part
+    o64, o32 = ref(O.ifft, f, norm=norm)
+    assert_2x(ops.ifft(f, norm=norm), o64, o32, f"ifft(real) {shape} {norm}")
+    c = ops.fft_complex(f.to(torch.complex64), norm="numpy_compatible")
+    o64, o32 = ref(O.fft_complex, f.to(torch.complex64), norm="numpy_compatible")
+    assert_2x(_ri(c), _ri(o64), _ri(o32), f"fft_complex {shape}")
+
+
+@pytest.mark.parametrize("shape,dx", [((16, 18, 20), 1.0), ((9, 14, 11), (0.5, 1.0, 0.25)), ((24, 30), (0.1, 0.2)),
+                                      ((50,), 0.3)])
+def test_spectral_derivatives(shape, dx):
+    base = field((1, 1) + shape if len(shape) == 1 else ((1,) + shape if len(shape) == 2 else shape), 42)
+    f = base.reshape(shape).contiguous()
+    for dim in range(len(shape)):
+        o64, o32 = ref(O.spectral_gradient, f, dx=dx, dim=dim)
+        assert_2x(ops.spectral_gradient(f, dx=dx, dim=dim), o64, o32, f"spectral_gradient {shape} dim={dim}")
+    o64, o32 = ref(O.spectral_laplacian, f, dx=dx)
+    assert_2x(ops.spectral_laplacian(f, dx=dx), o64, o32, f"spectral_laplacian {shape}")
+
+
+def test_spectral_ops_in_graphs_and_errors():
+    from oracle import graph_eval
+    shape = (12, 16, 14)
+    f = field(shape, 43)
+    for expr in ["spectral_laplacian(f) + laplacian(f)", "spectral_gradient(f, dim=2) * f", "ifft(fft(f)) - f"]:
+        g, sink, *_ = sp.compile_expression(expr, boundary="periodic", field_map={"f": f})
+        rt = sp.Runtime(device="cuda")
+        out = rt.run(g)[sink]
+        o64 = graph_eval.evaluate(g, torch.float64)[sink]
+        o32 = graph_eval.evaluate(g, torch.float32)[sink]
+        assert_2x(out, o64, o32, expr, slack=1e-12)
+    with pytest.raises(ValueError, match="1-D, 2-D or 3-D"):
+        ops.fft(torch.zeros((2, 2, 2, 2), device="cuda"))
+    with pytest.raises(IndexError):
+        ops.spectral_gradient(f, dim=3)

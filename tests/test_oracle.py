@@ -408,3 +408,48 @@ def test_conv3d_spectral_is_the_periodic_27_point_blur():
     f = torch.cos(th[0] * z + th[1] * y + th[2] * x)
     sym = math.prod((1 + math.cos(t)) / 2 for t in th)
     torch.testing.assert_close(O.conv3d_spectral(f), sym * f, rtol=1e-12, atol=1e-12)
+
+
+# ----------------------------------------------------------------------------- spectral
+def test_fft_oracle_matches_numpy_fft_for_every_norm():
+    rng = np.random.default_rng(5)
+    t = torch.from_numpy(rng.standard_normal((6, 7, 10)))
+    for norm in ("backward", "ortho", "forward"):
+        ref = np.fft.fftn(t.numpy(), norm=norm)
+        assert np.allclose(O.fft(t, norm=norm).numpy(), ref, rtol=0, atol=1e-12)
+        back = O.ifft(torch.from_numpy(ref), norm=norm)
+        assert back.dtype == torch.float64 and torch.allclose(back, t, atol=1e-12)
+        assert np.allclose(O.ifft_complex(torch.from_numpy(ref), norm=norm).numpy(),
+                           np.fft.ifftn(ref, norm=norm), atol=1e-12)
+    assert torch.equal(O.fft_complex(t.to(torch.complex128), norm="numpy_compatible"), O.fft(t))
+
+
+def test_wave_numbers_are_fp32_fftfreq_widened():
+    # the device kernel forms k = 2 pi * float(j) * float(1 / (n dx)); the oracle uses
+    # torch.fft.fftfreq itself: both must agree bit for bit
+    for n in (1, 2, 7, 8, 33, 64, 255, 256):
+        for dx in (1.0, 0.5, 0.3, 2.0 / 3):
+            grids, _ = O._k_grids((n,), (dx,), torch.float64)
+            j = np.arange(n)
+            j = np.where(j < (n + 1) // 2, j, j - n).astype(np.float32)
+            f = (j * np.float32(1.0 / (n * dx))).astype(np.float32)
+            k = f.astype(np.float64) * (2 * math.pi)
+            assert np.array_equal(grids[0].numpy(), k), (n, dx)
+
+
+def test_spectral_derivatives_of_fourier_modes_are_exact():
+    # below the Nyquist frequency spectral differentiation is exact: d/dx cos(2 pi m x / L)
+    shape, dxs = (8, 12, 10), (0.5, 1.0, 0.25)
+    idx = torch.meshgrid(*[torch.arange(n, dtype=torch.float64) for n in shape], indexing="ij")
+    modes = (1, 3, 2)
+    ph = [2 * math.pi * m * i / n for m, i, n in zip(modes, idx, shape)]
+    f = torch.cos(ph[0]) + torch.cos(ph[1] + 0.3) + torch.sin(ph[2])
+    om = [2 * math.pi * m / (n * d) for m, n, d in zip(modes, shape, dxs)]
+    exact = [-om[0] * torch.sin(ph[0]), -om[1] * torch.sin(ph[1] + 0.3), om[2] * torch.cos(ph[2])]
+    for d in range(3):
+        assert torch.allclose(O.spectral_gradient(f, dx=dxs, dim=d), exact[d], atol=1e-12), d
+    lap = -(om[0] ** 2) * torch.cos(ph[0]) - om[1] ** 2 * torch.cos(ph[1] + 0.3) - om[2] ** 2 * torch.sin(ph[2])
+    assert torch.allclose(O.spectral_laplacian(f, dx=dxs), lap, atol=1e-11)
+    # the Nyquist mode cos(pi i) has a purely imaginary derivative spectrum: its real part is 0
+    g = torch.cos(math.pi * idx[0])
+    assert O.spectral_gradient(g, dim=0).abs().max().item() < 1e-12
