@@ -1,8 +1,9 @@
 // Spectral operators (stenpy_engine.py:2871-3012): n-D FFTs and spectral derivatives.
 //
-// The transforms run through rocFFT in double precision (complex-to-complex, in place, on a
-// stream-ordered FP64 workspace); the hand-written kernels around them promote the FP32 input,
-// apply the spectral multiplier and round the result once to FP32.  The multiplier restates
+// The transforms run through rocFFT in double precision on a stream-ordered FP64 workspace:
+// complex-to-complex in place for the FFT ops, real-to-complex / complex-to-real (half the
+// work) for the spectral derivatives.  The hand-written kernels around them promote the FP32
+// input, apply the spectral multiplier and round the result once to FP32.  The multiplier restates
 // the reference's wave numbers exactly: k = 2 pi * fftfreq(n, dx) where fftfreq is evaluated in
 // FP32 (torch's default dtype) as float(j) * float(1 / (n dx)), j = i or i - n, and then
 // widened to FP64 (stenpy_engine.py:2939-2946).
@@ -24,13 +25,13 @@ constexpr double kTwoPi = 6.283185307179586;  // 2 * math.pi
 struct PlanKey {
     int ndim;
     size_t len[3];
-    int inverse;
+    int type;       // rocfft_transform_type
     int device;
     bool operator<(const PlanKey& o) const {
         if (ndim != o.ndim) return ndim < o.ndim;
         for (int i = 0; i < 3; ++i)
             if (len[i] != o.len[i]) return len[i] < o.len[i];
-        if (inverse != o.inverse) return inverse < o.inverse;
+        if (type != o.type) return type < o.type;
         return device < o.device;
     }
 };
@@ -49,12 +50,13 @@ int rocfft_check(rocfft_status s, const char* what) {
     return SP_OK;
 }
 
-// Plan of an in-place double-precision complex transform over all axes (cached per shape,
-// direction and device; plans live until the process ends).
-int get_plan(int ndim, const int64_t* shape, int inverse, Plan* out) {
+// Double-precision plan over all axes of the given type: complex forward / inverse in place,
+// real forward / inverse out of place (cached per shape, type and device; plans live until the
+// process ends).
+int get_plan(int ndim, const int64_t* shape, rocfft_transform_type type, Plan* out) {
     int dev = 0;
     SP_HIP_CHECK(hipGetDevice(&dev));
-    PlanKey key{ndim, {1, 1, 1}, inverse, dev};
+    PlanKey key{ndim, {1, 1, 1}, (int)type, dev};
     for (int i = 0; i < ndim; ++i) key.len[i] = (size_t)shape[ndim - 1 - i];  // fastest first
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_setup) {
@@ -64,10 +66,10 @@ int get_plan(int ndim, const int64_t* shape, int inverse, Plan* out) {
     auto it = g_plans.find(key);
     if (it == g_plans.end()) {
         Plan p;
-        if (int rc = rocfft_check(rocfft_plan_create(&p.plan, rocfft_placement_inplace,
-                                                     inverse ? rocfft_transform_type_complex_inverse
-                                                             : rocfft_transform_type_complex_forward,
-                                                     rocfft_precision_double, (size_t)ndim, key.len, 1,
+        const bool cplx = type == rocfft_transform_type_complex_forward ||
+                          type == rocfft_transform_type_complex_inverse;
+        if (int rc = rocfft_check(rocfft_plan_create(&p.plan, cplx ? rocfft_placement_inplace : rocfft_placement_notinplace,
+                                                     type, rocfft_precision_double, (size_t)ndim, key.len, 1,
                                                      nullptr),
                                   "rocfft_plan_create"))
             return rc;
@@ -78,15 +80,16 @@ int get_plan(int ndim, const int64_t* shape, int inverse, Plan* out) {
     return SP_OK;
 }
 
-int run_plan(const Plan& p, double2* z, hipStream_t st) {
+int run_plan(const Plan& p, void* in, void* out, hipStream_t st) {
     void* work = nullptr;
     if (p.work) SP_HIP_CHECK(hipMallocAsync(&work, p.work, st));
     rocfft_execution_info info = nullptr;
     int rc = rocfft_check(rocfft_execution_info_create(&info), "execution info");
     if (!rc) rc = rocfft_check(rocfft_execution_info_set_stream(info, st), "set stream");
     if (!rc && p.work) rc = rocfft_check(rocfft_execution_info_set_work_buffer(info, work, p.work), "work buffer");
-    void* bufs[1] = {z};
-    if (!rc) rc = rocfft_check(rocfft_execute(p.plan, bufs, nullptr, info), "rocfft_execute");
+    void* ibuf[1] = {in};
+    void* obuf[1] = {out};
+    if (!rc) rc = rocfft_check(rocfft_execute(p.plan, ibuf, out ? obuf : nullptr, info), "rocfft_execute");
     if (info) rocfft_execution_info_destroy(info);
     if (work) {
         hipError_t e = hipFreeAsync(work, st);
@@ -117,24 +120,42 @@ __global__ void k_store(const double2* __restrict__ z, float* __restrict__ out, 
 
 struct KAxes {
     int64_t n[3];   // as a 3-D field (leading axes of length 1 for 1-D / 2-D inputs)
+    int64_t h2;     // stored length of the last axis: n[2] / 2 + 1 (half spectrum)
     float s[3];     // float(1 / (n dx)) per axis
 };
+
+// FP32 real -> FP64 real
+__global__ void k_load_real(const float* __restrict__ x, double* __restrict__ r, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        r[i] = (double)x[i];
+}
+
+__global__ void k_store_real(const double* __restrict__ r, float* __restrict__ out, double scale, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (float)(r[i] * scale);
+}
 
 __device__ __forceinline__ double wave_number(int64_t i, int64_t n, float s) {
     const int64_t j = i < (n + 1) / 2 ? i : i - n;
     return __dmul_rn((double)__fmul_rn((float)j, s), kTwoPi);
 }
 
-// z *= i k_dim (kind 0) or z *= -(k0^2 + k1^2 + k2^2) (kind 1), products rounded separately
-// as torch evaluates them (no contraction)
+// On the half spectrum (n0, n1, n2 / 2 + 1) of a real field: z *= i k_dim (kind 0) or
+// z *= -(k0^2 + k1^2 + k2^2) (kind 1), products rounded separately as torch evaluates them (no
+// contraction).  For kind 0 the Nyquist plane of axis dim (even length) is zeroed: there
+// i k_dim X is anti-Hermitian, so its share of ifftn is purely imaginary and the reference's
+// .real drops it; everywhere else the product is Hermitian and the complex-to-real inverse
+// returns exactly that real part.
 __global__ void k_kmul(double2* __restrict__ z, const KAxes ax, int kind, int dim, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i2 = i % ax.n[2], i1 = (i / ax.n[2]) % ax.n[1], i0 = i / (ax.n[2] * ax.n[1]);
+        const int64_t i2 = i % ax.h2, i1 = (i / ax.h2) % ax.n[1], i0 = i / (ax.h2 * ax.n[1]);
         const double2 v = z[i];
         if (kind == 0) {
             const int64_t ii = dim == 0 ? i0 : (dim == 1 ? i1 : i2);
-            const double k = wave_number(ii, ax.n[dim], ax.s[dim]);
-            z[i] = make_double2(-__dmul_rn(k, v.y), __dmul_rn(k, v.x));
+            const int64_t nd = ax.n[dim];
+            const double k = wave_number(ii, nd, ax.s[dim]);
+            z[i] = ((nd & 1) == 0 && ii == nd / 2) ? make_double2(0.0, 0.0)
+                                                     : make_double2(-__dmul_rn(k, v.y), __dmul_rn(k, v.x));
         } else {
             const double k0 = wave_number(i0, ax.n[0], ax.s[0]);
             const double k1 = wave_number(i1, ax.n[1], ax.s[1]);
@@ -178,12 +199,14 @@ extern "C" int sp_fft(const sp_tensor* x, sp_tensor* out, int inverse, int norm,
     if (n == 0) return SP_OK;
     hipStream_t st = as_stream(stream);
     Plan plan;
-    if (int rc = get_plan(x->ndim, x->shape, inverse, &plan)) return rc;
+    if (int rc = get_plan(x->ndim, x->shape, inverse ? rocfft_transform_type_complex_inverse
+                                                     : rocfft_transform_type_complex_forward, &plan))
+        return rc;
     double2* z = nullptr;
     SP_HIP_CHECK(hipMallocAsync((void**)&z, sizeof(double2) * n, st));
     k_load<<<grid_for(n), 256, 0, st>>>((const float*)x->ptr, x->dtype == SP_C64, z, n);
     int rc = launch_status("fft load");
-    if (!rc) rc = run_plan(plan, z, st);
+    if (!rc) rc = run_plan(plan, z, nullptr, st);
     if (!rc) {
         // torch.fft norm: "backward" scales the inverse by 1/n, "forward" the forward one,
         // "ortho" both by 1/sqrt(n)
@@ -217,26 +240,36 @@ extern "C" int sp_spectral(const sp_tensor* x, sp_tensor* out, const double* dx,
         SP_REQUIRE(d != 0.0, "sp_spectral: zero spacing");
         ax.s[a] = (float)(1.0 / ((double)ax.n[a] * d));
     }
+    ax.h2 = ax.n[2] / 2 + 1;
+    const int64_t nh = ax.n[0] * ax.n[1] * ax.h2;
     hipStream_t st = as_stream(stream);
     Plan fwd, inv;
-    if (int rc = get_plan(nd, x->shape, 0, &fwd)) return rc;
-    if (int rc = get_plan(nd, x->shape, 1, &inv)) return rc;
+    if (int rc = get_plan(nd, x->shape, rocfft_transform_type_real_forward, &fwd)) return rc;
+    if (int rc = get_plan(nd, x->shape, rocfft_transform_type_real_inverse, &inv)) return rc;
+    double* r = nullptr;
     double2* z = nullptr;
-    SP_HIP_CHECK(hipMallocAsync((void**)&z, sizeof(double2) * n, st));
-    k_load<<<grid_for(n), 256, 0, st>>>((const float*)x->ptr, 0, z, n);
-    int rc = launch_status("spectral load");
-    if (!rc) rc = run_plan(fwd, z, st);
+    SP_HIP_CHECK(hipMallocAsync((void**)&r, sizeof(double) * n, st));
+    hipError_t ea = hipMallocAsync((void**)&z, sizeof(double2) * nh, st);
+    int rc = ea == hipSuccess ? SP_OK : fail(SP_EHIP, std::string("hipMallocAsync: ") + hipGetErrorString(ea));
     if (!rc) {
-        k_kmul<<<grid_for(n), 256, 0, st>>>(z, ax, kind, dim + (3 - nd), n);
+        k_load_real<<<grid_for(n), 256, 0, st>>>((const float*)x->ptr, r, n);
+        rc = launch_status("spectral load");
+    }
+    if (!rc) rc = run_plan(fwd, r, z, st);
+    if (!rc) {
+        k_kmul<<<grid_for(nh), 256, 0, st>>>(z, ax, kind, dim + (3 - nd), nh);
         rc = launch_status("spectral multiply");
     }
-    if (!rc) rc = run_plan(inv, z, st);
+    if (!rc) rc = run_plan(inv, z, r, st);
     if (!rc) {
         // forward and inverse with the same norm scale by 1/n in total, whatever the norm
-        k_store<<<grid_for(n), 256, 0, st>>>(z, (float*)out->ptr, 1, 1.0 / (double)n, n);
+        k_store_real<<<grid_for(n), 256, 0, st>>>(r, (float*)out->ptr, 1.0 / (double)n, n);
         rc = launch_status("spectral store");
     }
-    hipError_t e = hipFreeAsync(z, st);
-    if (!rc && e != hipSuccess) rc = fail(SP_EHIP, std::string("hipFreeAsync: ") + hipGetErrorString(e));
+    for (void* b : {(void*)z, (void*)r}) {
+        if (!b) continue;
+        hipError_t e = hipFreeAsync(b, st);
+        if (!rc && e != hipSuccess) rc = fail(SP_EHIP, std::string("hipFreeAsync: ") + hipGetErrorString(e));
+    }
     return rc;
 }
