@@ -116,6 +116,13 @@ int sp_fft(const sp_tensor* x, sp_tensor* out, int inverse, int norm, void* stre
 enum { SP_SPECTRAL_GRADIENT = 0, SP_SPECTRAL_LAPLACIAN = 1 };
 int sp_spectral(const sp_tensor* x, sp_tensor* out, const double* dx, int kind, int dim, void* stream);
 
+/* ---- geometry (stenpy_engine.py:3115-3156) ------------------------------------------- */
+/* Exact Euclidean distance transform with per-axis sampling dx[ndim]: the CPU branch,
+ * scipy.ndimage.distance_transform_edt(t == 0, sampling=dx).  Voxels with t != 0 get 0, every
+ * voxel with t == 0 the distance to the nearest voxel with t != 0 (+inf when there is none).
+ * FP64 internally, FP32 out; x, out contiguous 1-D to 3-D. */
+int sp_distance_transform(const sp_tensor* x, sp_tensor* out, const double* dx, void* stream);
+
 /* 3-D single-pass kernels for scalar (n0,n1,n2) / vector (n0,n1,n2,3) fields.
  * Each replaces the CPU branch of the cited reference op (the parity target). */
 /* laplacian, stenpy_engine.py:1478-1512 */

@@ -14,8 +14,9 @@ select_component(G,c)`` (left fold) and ``sum_c select_component(C,c)**2``.
 How each function is pinned (tests/test_oracle.py): periodic stencils by the discrete Fourier
 symbols of single modes, one-sided edges by exact derivatives of polynomials of degree <= 2,
 hessian/tensor ops by the quadratic i^2+j^2+k^2 (H = 2I) and by numpy.linalg, reductions by
-math.fsum / numpy, Simpson by exact polynomial integrals, and everything else by brute-force
-loops on tiny grids written from the formulas.
+math.fsum / numpy, Simpson by exact polynomial integrals, the FFT ops by numpy.fft and the exact
+spectral derivatives of Fourier modes, and everything else (the distance transform included) by
+brute-force loops on tiny grids written from the formulas.
 """
 from __future__ import annotations
 
@@ -515,6 +516,14 @@ def spectral_laplacian(t: Tensor, dx: Any = 1.0, norm: str = "backward") -> Tens
     return torch.fft.ifftn(-k2 * T_hat, norm=norm).real.to(t.dtype)
 
 
+# ----------------------------------------------------------------------------- geometry
+def distance_transform(t: Tensor, dx: Any = 1.0) -> Tensor:           # 3115-3156, CPU branch
+    from scipy.ndimage import distance_transform_edt
+    dxs = (float(dx),) * t.ndim if isinstance(dx, (int, float)) else tuple(dx)
+    out = distance_transform_edt((t == 0).numpy(), sampling=list(dxs))
+    return torch.from_numpy(out.astype("float64")).to(t.dtype)
+
+
 OPS = {
     "add": add, "sub": sub, "mul": mul, "div": div, "neg": neg, "clamp": clamp, "exp": exp,
     "log": log, "sqrt": sqrt, "sin": sin, "cos": cos, "tanh": tanh,
@@ -533,4 +542,5 @@ OPS = {
     "inverse": inverse, "deviatoric": deviatoric,
     "fft": fft, "ifft": ifft, "fft_complex": fft_complex, "ifft_complex": ifft_complex,
     "spectral_gradient": spectral_gradient, "spectral_laplacian": spectral_laplacian,
+    "distance_transform": distance_transform,
 }

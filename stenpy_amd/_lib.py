@@ -77,6 +77,7 @@ def _declare(lib: C.CDLL) -> None:
         "sp_axis_fir": ([P, I, P, I, C.POINTER(D), I, V], I),
         "sp_fft": ([P, P, I, I, V], I),
         "sp_spectral": ([P, P, C.POINTER(D), I, I, V], I),
+        "sp_distance_transform": ([P, P, C.POINTER(D), V], I),
         "sp_axis_pass": ([P, I, P, I, I, D, D, I, V], I),
         "sp_laplacian3d": ([P, P, C.POINTER(D), I, V], I),
         "sp_gradient3d": ([P, P, D, I, V], I),
@@ -226,6 +227,14 @@ def fft(x: torch.Tensor, out: torch.Tensor, inverse: bool, norm: str) -> torch.T
         raise ValueError(f"unknown FFT norm {norm!r} (backward, ortho or forward)")
     check(lib().sp_fft(C.byref(desc(x)), C.byref(desc(out)), int(bool(inverse)), NORM[norm],
                        stream_ptr(x.device)), "fft")
+    return out
+
+
+def distance_transform(x: torch.Tensor, out: torch.Tensor, dx) -> torch.Tensor:
+    require_gpu(x, out)
+    d = (C.c_double * x.ndim)(*[float(v) for v in dx])
+    check(lib().sp_distance_transform(C.byref(desc(x)), C.byref(desc(out)), d, stream_ptr(x.device)),
+          "distance transform")
     return out
 
 

@@ -450,6 +450,22 @@ register_operator("spectral_gradient", spectral_gradient, cost="high")
 register_operator("spectral_laplacian", spectral_laplacian, cost="high")
 
 
+# ----------------------------------------------------------------------------- geometry
+def distance_transform(t, dx: Any = 1.0, alloc=None, method: str = "auto"):
+    """Exact Euclidean distance transform of (t == 0) with per-axis sampling ``dx``: 0 where
+    t != 0, elsewhere the distance to the nearest voxel with t != 0 -- the reference's CPU
+    branch, scipy's distance_transform_edt (stenpy_engine.py:3115-3156; its Triton JFA branch
+    seeds the opposite set and is approximate).  A field without a nonzero voxel gives +inf."""
+    dev = _device_of(t)
+    t = _contig(_f32(t, dev))
+    if t.ndim > 3:
+        raise NotImplementedError("distance_transform: only 1-D/2-D/3-D supported")
+    return L.distance_transform(t, _empty(t.shape, dev), _dxs(dx, t.ndim))
+
+
+register_operator("distance_transform", distance_transform, cost="very_high")
+
+
 # ----------------------------------------------------------------------------- channel-last helpers
 def stack_components(*tensors, dim: int = -1):
     dev = _device_of(*tensors)

@@ -303,3 +303,31 @@ def test_spectral_ops_in_graphs_and_errors():
         ops.fft(torch.zeros((2, 2, 2, 2), device="cuda"))
     with pytest.raises(IndexError):
         ops.spectral_gradient(f, dim=3)
+
+
+# ----------------------------------------------------------------------------- geometry
+@pytest.mark.parametrize("shape,dx,density", [((20, 24, 28), 1.0, 0.3), ((17, 9, 33), (0.5, 1.0, 2.0), 0.05),
+                                              ((40, 36), (0.3, 0.7), 0.5), ((64,), 0.25, 0.1),
+                                              ((12, 14, 16), 1.0, 0.002)])
+def test_distance_transform(shape, dx, density):
+    g = torch.Generator().manual_seed(7)
+    keep = torch.rand(shape, generator=g) < density
+    t = torch.where(keep, torch.randn(shape, generator=g), torch.zeros(shape))
+    t.view(-1)[0] = 1.0            # at least one voxel with t != 0
+    tc = t.float().cuda()
+    out = ops.distance_transform(tc, dx=dx)
+    o64, o32 = ref(O.distance_transform, tc, dx=dx)
+    assert_2x(out, o64, o32, f"distance_transform {shape} {dx}")
+    assert torch.equal(out.cpu() == 0, t != 0)
+
+
+def test_distance_transform_in_graphs_and_empty_background():
+    from oracle import graph_eval
+    f = field((18, 20, 22), 44)
+    g, sink, *_ = sp.compile_expression("distance_transform(f - clamp(f, lo=-0.05, hi=0.05))", boundary="periodic",
+                                        field_map={"f": f})
+    out = sp.Runtime(device="cuda").run(g)[sink]
+    o64 = graph_eval.evaluate(g, torch.float64)[sink]
+    o32 = graph_eval.evaluate(g, torch.float32)[sink]
+    assert_2x(out, o64, o32, "distance_transform graph")
+    assert torch.isinf(ops.distance_transform(torch.zeros((4, 5, 6), device="cuda"))).all()

@@ -453,3 +453,27 @@ def test_spectral_derivatives_of_fourier_modes_are_exact():
     # the Nyquist mode cos(pi i) has a purely imaginary derivative spectrum: its real part is 0
     g = torch.cos(math.pi * idx[0])
     assert O.spectral_gradient(g, dim=0).abs().max().item() < 1e-12
+
+
+# ----------------------------------------------------------------------------- geometry
+def test_distance_transform_oracle_matches_brute_force():
+    rng = np.random.default_rng(11)
+    for trial in range(12):
+        shape = tuple(int(v) for v in rng.integers(1, 7, 3))
+        t = torch.from_numpy((rng.random(shape) < 0.3) * rng.standard_normal(shape))
+        if not (t != 0).any():
+            t.view(-1)[0] = 1.0
+        s = tuple(float(v) for v in rng.choice([0.5, 1.0, 2.0, 0.3], 3))
+        got = O.distance_transform(t, dx=s)
+        bg = np.argwhere(t.numpy() != 0)
+        want = np.zeros(shape)
+        for idx in np.ndindex(*shape):
+            if t[idx] == 0:
+                want[idx] = np.sqrt((((bg - np.array(idx)) * np.array(s)) ** 2).sum(1)).min()
+        assert np.allclose(got.numpy(), want, rtol=0, atol=1e-12), (shape, s)
+    # one background voxel: the exact Euclidean distance to it, and 0 on it
+    t = torch.zeros((5, 6, 7), dtype=torch.float64)
+    t[2, 3, 4] = -1.5
+    i, j, k = torch.meshgrid(*[torch.arange(n, dtype=torch.float64) for n in t.shape], indexing="ij")
+    want = torch.sqrt(((i - 2) * 0.5) ** 2 + (j - 3) ** 2 + ((k - 4) * 2.0) ** 2)
+    assert torch.allclose(O.distance_transform(t, dx=(0.5, 1.0, 2.0)), want, atol=1e-12)
