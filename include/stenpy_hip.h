@@ -116,6 +116,24 @@ int sp_fft(const sp_tensor* x, sp_tensor* out, int inverse, int norm, void* stre
 enum { SP_SPECTRAL_GRADIENT = 0, SP_SPECTRAL_LAPLACIAN = 1 };
 int sp_spectral(const sp_tensor* x, sp_tensor* out, const double* dx, int kind, int dim, void* stream);
 
+/* ---- statistics and integrals (CPU branches of stenpy_engine.py) ------------------------ */
+/* Every sum is accumulated in FP64 from FP32 inputs; global results are written to ``result``,
+ * one double in device memory, with no host synchronisation. */
+/* cumulative_integral, 2462-2483: out[k] = sum_{j<k} (x[j] + x[j+1]) * dx / 2 along ``dim``,
+ * out[0] = 0; x, out FP32 of one shape. */
+int sp_cumulative_trapezoid(const sp_tensor* x, sp_tensor* out, int dim, double dx, void* stream);
+/* surface_integral, 2486-2496: sum_i w[i] * sum_c t[i, c] * normals[i, c]; normals (..., C),
+ * area_weights (...), t (..., C) or (..., 1). */
+int sp_surface_integral(const sp_tensor* t, const sp_tensor* normals, const sp_tensor* weights,
+                        double* result, void* stream);
+/* covariance (2575-2592): dot(a - mean a, b - mean b) / max(n - 1, 1);
+ * correlation (2595-2615): dot(a', b') / (|a'| |b'| + 1e-15). */
+enum { SP_COVARIANCE = 0, SP_CORRELATION = 1 };
+int sp_pair_stat(int kind, const sp_tensor* a, const sp_tensor* b, double* result, void* stream);
+/* entropy, 2638-2669: -sum p log(p + eps), p = |x| / (sum |x| + eps); over everything into
+ * ``result`` (dim < 0), or along ``dim`` into ``out`` (FP32, the input's shape without dim). */
+int sp_entropy(const sp_tensor* x, int dim, double eps, double* result, sp_tensor* out, void* stream);
+
 /* ---- geometry (stenpy_engine.py:3115-3156) ------------------------------------------- */
 /* Exact Euclidean distance transform with per-axis sampling dx[ndim]: the CPU branch,
  * scipy.ndimage.distance_transform_edt(t == 0, sampling=dx).  Voxels with t != 0 get 0, every

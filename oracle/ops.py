@@ -516,6 +516,32 @@ def spectral_laplacian(t: Tensor, dx: Any = 1.0, norm: str = "backward") -> Tens
     return torch.fft.ifftn(-k2 * T_hat, norm=norm).real.to(t.dtype)
 
 
+# ----------------------------------------------------------------------------- statistics
+def surface_integral(t: Tensor, normals: Tensor, area_weights: Tensor) -> Tensor:   # 2486-2496
+    return ((t * normals).sum(dim=-1) * area_weights).sum()
+
+
+def covariance(a: Tensor, b: Tensor) -> Tensor:                    # 2575-2592, CPU branch
+    af = (a - a.mean()).flatten()
+    bf = (b - b.mean()).flatten()
+    return torch.dot(af, bf) / max(af.numel() - 1, 1)
+
+
+def correlation(a: Tensor, b: Tensor) -> Tensor:                   # 2595-2615, CPU branch
+    af = (a - a.mean()).flatten()
+    bf = (b - b.mean()).flatten()
+    return torch.dot(af, bf) / (af.norm() * bf.norm() + 1e-15)
+
+
+def entropy(t: Tensor, dim=None, eps: float = 1e-15) -> Tensor:   # 2638-2669, CPU branch
+    p = t.abs()
+    if dim is None:
+        p = p / (p.sum() + eps)
+        return -(p * (p + eps).log()).sum()
+    p = p / (p.sum(dim=dim, keepdim=True) + eps)
+    return -(p * (p + eps).log()).sum(dim=dim)
+
+
 # ----------------------------------------------------------------------------- geometry
 def distance_transform(t: Tensor, dx: Any = 1.0) -> Tensor:           # 3115-3156, CPU branch
     from scipy.ndimage import distance_transform_edt
@@ -543,4 +569,6 @@ OPS = {
     "fft": fft, "ifft": ifft, "fft_complex": fft_complex, "ifft_complex": ifft_complex,
     "spectral_gradient": spectral_gradient, "spectral_laplacian": spectral_laplacian,
     "distance_transform": distance_transform,
+    "surface_integral": surface_integral, "covariance": covariance, "correlation": correlation,
+    "entropy": entropy,
 }

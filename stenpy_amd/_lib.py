@@ -78,6 +78,10 @@ def _declare(lib: C.CDLL) -> None:
         "sp_fft": ([P, P, I, I, V], I),
         "sp_spectral": ([P, P, C.POINTER(D), I, I, V], I),
         "sp_distance_transform": ([P, P, C.POINTER(D), V], I),
+        "sp_cumulative_trapezoid": ([P, P, I, D, V], I),
+        "sp_surface_integral": ([P, P, P, V, V], I),
+        "sp_pair_stat": ([I, P, P, V, V], I),
+        "sp_entropy": ([P, I, D, V, V, V], I),
         "sp_axis_pass": ([P, I, P, I, I, D, D, I, V], I),
         "sp_laplacian3d": ([P, P, C.POINTER(D), I, V], I),
         "sp_gradient3d": ([P, P, D, I, V], I),
@@ -227,6 +231,45 @@ def fft(x: torch.Tensor, out: torch.Tensor, inverse: bool, norm: str) -> torch.T
         raise ValueError(f"unknown FFT norm {norm!r} (backward, ortho or forward)")
     check(lib().sp_fft(C.byref(desc(x)), C.byref(desc(out)), int(bool(inverse)), NORM[norm],
                        stream_ptr(x.device)), "fft")
+    return out
+
+
+def cumulative_trapezoid(x: torch.Tensor, out: torch.Tensor, dim: int, dx: float) -> torch.Tensor:
+    require_gpu(x, out)
+    check(lib().sp_cumulative_trapezoid(C.byref(desc(x)), C.byref(desc(out)), int(dim), float(dx),
+                                        stream_ptr(x.device)), "cumulative integral")
+    return out
+
+
+def surface_integral(t: torch.Tensor, normals: torch.Tensor, weights: torch.Tensor) -> torch.Tensor:
+    require_gpu(t, normals, weights)
+    res = torch.empty((), dtype=torch.float64, device=t.device)
+    check(lib().sp_surface_integral(C.byref(desc(t)), C.byref(desc(normals)), C.byref(desc(weights)),
+                                    C.c_void_p(res.data_ptr()), stream_ptr(t.device)), "surface integral")
+    return res
+
+
+PAIR_STAT = {"covariance": 0, "correlation": 1}
+
+
+def pair_stat(kind: str, a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    require_gpu(a, b)
+    res = torch.empty((), dtype=torch.float64, device=a.device)
+    check(lib().sp_pair_stat(PAIR_STAT[kind], C.byref(desc(a)), C.byref(desc(b)), C.c_void_p(res.data_ptr()),
+                             stream_ptr(a.device)), kind)
+    return res
+
+
+def entropy(x: torch.Tensor, dim: Optional[int], eps: float) -> torch.Tensor:
+    require_gpu(x)
+    if dim is None:
+        res = torch.empty((), dtype=torch.float64, device=x.device)
+        check(lib().sp_entropy(C.byref(desc(x)), -1, float(eps), C.c_void_p(res.data_ptr()), None,
+                               stream_ptr(x.device)), "entropy")
+        return res
+    out = torch.empty(x.shape[:dim] + x.shape[dim + 1:], dtype=torch.float32, device=x.device)
+    check(lib().sp_entropy(C.byref(desc(x)), int(dim), float(eps), None, C.byref(desc(out)),
+                           stream_ptr(x.device)), "entropy")
     return out
 
 

@@ -450,6 +450,68 @@ register_operator("spectral_gradient", spectral_gradient, cost="high")
 register_operator("spectral_laplacian", spectral_laplacian, cost="high")
 
 
+# ----------------------------------------------------------------------------- statistics and integrals
+def cumulative_integral(t, dx: float = 1.0, dim: int = 0, alloc=None):
+    """Cumulative trapezoid along ``dim`` with a leading 0, accumulated in FP64
+    (stenpy_engine.py:2462-2483)."""
+    dev = _device_of(t)
+    t = _contig(_f32(t, dev))
+    dim = int(dim)
+    if not -t.ndim <= dim < t.ndim:
+        raise IndexError(f"cumulative_integral: dim {dim} out of range for a {t.ndim}-D field")
+    return L.cumulative_trapezoid(t, _empty(t.shape, dev), dim % t.ndim, float(dx))
+
+
+def surface_integral(t, normals, area_weights, alloc=None):
+    """sum_i w_i sum_c t[i, c] normals[i, c] in FP64 (stenpy_engine.py:2486-2496).  ``t`` has the
+    normals' shape or a last axis of length 1; ``area_weights`` the normals' shape without the
+    last axis."""
+    dev = _device_of(t, normals, area_weights)
+    t, n, w = (_contig(_f32(v, dev)) for v in (t, normals, area_weights))
+    if n.ndim < 1 or w.shape != n.shape[:-1] or t.shape not in (n.shape, n.shape[:-1] + (1,)):
+        raise ValueError(f"surface_integral: shapes t {tuple(t.shape)}, normals {tuple(n.shape)}, "
+                         f"area_weights {tuple(w.shape)} are not supported")
+    return L.surface_integral(t, n, w)
+
+
+def _pair(kind: str, a, b):
+    dev = _device_of(a, b)
+    a, b = _contig(_f32(a, dev)), _contig(_f32(b, dev))
+    if a.shape != b.shape:
+        raise ValueError(f"{kind}: shape mismatch {tuple(a.shape)} vs {tuple(b.shape)}")
+    return L.pair_stat(kind, a, b)
+
+
+def covariance(a, b, alloc=None):
+    """dot(a - mean a, b - mean b) / max(n - 1, 1), FP64 (stenpy_engine.py:2575-2592)."""
+    return _pair("covariance", a, b)
+
+
+def correlation(a, b, alloc=None):
+    """dot(a', b') / (|a'| |b'| + 1e-15) of the centred fields, FP64 (stenpy_engine.py:2595-2615)."""
+    return _pair("correlation", a, b)
+
+
+def entropy(t, dim=None, eps: float = 1e-15, alloc=None):
+    """-sum p log(p + eps), p = |t| / (sum |t| + eps): over everything (FP64 scalar) or along
+    ``dim`` (stenpy_engine.py:2638-2669)."""
+    dev = _device_of(t)
+    t = _contig(_f32(t, dev))
+    if dim is not None:
+        dim = int(dim)
+        if not -t.ndim <= dim < t.ndim:
+            raise IndexError(f"entropy: dim {dim} out of range for a {t.ndim}-D field")
+        dim %= t.ndim
+    return L.entropy(t, dim, float(eps))
+
+
+register_operator("cumulative_integral", cumulative_integral, cost="low")
+register_operator("surface_integral", surface_integral, cost="medium")
+register_operator("covariance", covariance, cost="medium")
+register_operator("correlation", correlation, cost="medium")
+register_operator("entropy", entropy, cost="medium")
+
+
 # ----------------------------------------------------------------------------- geometry
 def distance_transform(t, dx: Any = 1.0, alloc=None, method: str = "auto"):
     """Exact Euclidean distance transform of (t == 0) with per-axis sampling ``dx``: 0 where

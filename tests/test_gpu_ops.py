@@ -331,3 +331,46 @@ def test_distance_transform_in_graphs_and_empty_background():
     o32 = graph_eval.evaluate(g, torch.float32)[sink]
     assert_2x(out, o64, o32, "distance_transform graph")
     assert torch.isinf(ops.distance_transform(torch.zeros((4, 5, 6), device="cuda"))).all()
+
+
+# ----------------------------------------------------------------------------- statistics and integrals
+@pytest.mark.parametrize("shape,dim", [((12, 14, 16), 0), ((12, 14, 16), 1), ((12, 14, 16), 2), ((30, 1), 0), ((1, 9), 0)])
+def test_cumulative_integral(shape, dim):
+    f = field((1,) + shape if len(shape) == 2 else shape, 45).reshape(shape).contiguous()
+    o64, o32 = ref(O.cumulative_integral, f, dx=0.5, dim=dim)
+    assert_2x(ops.cumulative_integral(f, dx=0.5, dim=dim), o64, o32, f"cumulative_integral {shape} dim={dim}")
+
+
+def test_pair_statistics_entropy_and_surface_integral():
+    a, b = field((20, 24, 28), 46), field((20, 24, 28), 47)
+    b = 0.4 * a + b
+    for name in ("covariance", "correlation"):
+        out = getattr(ops, name)(a, b)
+        assert out.dtype == torch.float64 and out.ndim == 0
+        o64, o32 = ref(getattr(O, name), a, b)
+        assert_2x(out, o64, o32, name)
+    o64, o32 = ref(O.entropy, a)
+    assert_2x(ops.entropy(a), o64, o32, "entropy")
+    for d in range(3):
+        o64, o32 = ref(O.entropy, a, dim=d)
+        assert_2x(ops.entropy(a, dim=d), o64, o32, f"entropy dim={d}")
+    v, n = vfield((10, 12, 14)), vfield((10, 12, 14), seeds=(6, 7, 8))
+    w = field((10, 12, 14), 48).abs()
+    for t in (v, field((10, 12, 14), 49).unsqueeze(-1).contiguous()):
+        o64, o32 = ref(O.surface_integral, t, n, w)
+        assert_2x(ops.surface_integral(t, n, w), o64, o32, f"surface_integral t{tuple(t.shape)}")
+    with pytest.raises(ValueError):
+        ops.covariance(a, b[:-1])
+    with pytest.raises(ValueError):
+        ops.surface_integral(v, n, w[:-1])
+
+
+def test_statistics_in_graphs():
+    from oracle import graph_eval
+    f = field((14, 16, 18), 50)
+    for expr in ["covariance(f, tanh(f))", "correlation(f, f * f)", "entropy(f)", "cumulative_integral(f, dim=1) * f"]:
+        g, sink, *_ = sp.compile_expression(expr, boundary="periodic", field_map={"f": f})
+        out = sp.Runtime(device="cuda").run(g)[sink]
+        o64 = graph_eval.evaluate(g, torch.float64)[sink]
+        o32 = graph_eval.evaluate(g, torch.float32)[sink]
+        assert_2x(out, o64, o32, expr)

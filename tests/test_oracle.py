@@ -477,3 +477,25 @@ def test_distance_transform_oracle_matches_brute_force():
     i, j, k = torch.meshgrid(*[torch.arange(n, dtype=torch.float64) for n in t.shape], indexing="ij")
     want = torch.sqrt(((i - 2) * 0.5) ** 2 + (j - 3) ** 2 + ((k - 4) * 2.0) ** 2)
     assert torch.allclose(O.distance_transform(t, dx=(0.5, 1.0, 2.0)), want, atol=1e-12)
+
+
+# ----------------------------------------------------------------------------- statistics
+def test_statistics_oracle_against_numpy_and_scipy():
+    from scipy.stats import entropy as sp_entropy
+    rng = np.random.default_rng(12)
+    a = torch.from_numpy(rng.standard_normal((5, 6, 7)))
+    b = torch.from_numpy(0.3 * a.numpy() + rng.standard_normal((5, 6, 7)))
+    assert math.isclose(O.covariance(a, b).item(), np.cov(a.numpy().ravel(), b.numpy().ravel())[0, 1], rel_tol=1e-12)
+    assert math.isclose(O.correlation(a, b).item(), np.corrcoef(a.numpy().ravel(), b.numpy().ravel())[0, 1],
+                        rel_tol=1e-12)
+    assert math.isclose(O.entropy(a).item(), sp_entropy(np.abs(a.numpy().ravel())), rel_tol=1e-12)
+    per = O.entropy(a, dim=1).numpy()
+    want = np.apply_along_axis(lambda v: sp_entropy(np.abs(v)), 1, a.numpy())
+    assert np.allclose(per, want, rtol=1e-12)
+    t, n, w = (torch.from_numpy(rng.standard_normal(s)) for s in ((4, 5, 3), (4, 5, 3), (4, 5)))
+    brute = sum(w[i, j].item() * sum(t[i, j, c].item() * n[i, j, c].item() for c in range(3))
+                for i in range(4) for j in range(5))
+    assert math.isclose(O.surface_integral(t, n, w).item(), brute, rel_tol=1e-12)
+    # cumulative trapezoid: its last entry is numpy's trapezoid rule over the axis
+    assert np.allclose(O.cumulative_integral(a, dx=0.5, dim=2).numpy()[..., -1],
+                       np.trapezoid(a.numpy(), dx=0.5, axis=2), rtol=1e-12)
