@@ -974,7 +974,7 @@ class KernelGen:
         self._choose_modes()
         # outputs of >= 4 channels written through a per-wave LDS transpose (coalesced lines)
         self.stage = [oi for oi, o in enumerate(group.outputs)
-                      if STAGE_STORES and self.use_lds and not batched and sched.tx % 64 == 0
+                      if STAGE_STORES and self.use_lds and not batched and (sched.tx % 64 == 0 or 64 % sched.tx == 0)
                       and info[o].ch >= 4 and out_dtypes.get(o) != torch.bfloat16]
         # per wave: 64 * C floats per staged output, deposited by the lanes inside the variant
         self.stage_off: Dict[int, int] = {}
@@ -1701,21 +1701,26 @@ class KernelGen:
             A(f"{ind}      {self._variant_name(('gather',))}{call}")
         A(f"{ind}    }}")
         if staged:
-            # the wave's 64 voxels are consecutive in x (TX % 64 == 0): each lane deposited its C
-            # channels inside the variant; element q * 64 + l of the wave's C * 64 outputs is
-            # stored by lane l
+            # lane l of a wave holds voxel (y_0 + l / W, x_0 + l % W), W = min(TX, 64): one run of
+            # 64 consecutive voxels, or 64 / TX runs of TX on consecutive rows.  Each lane
+            # deposited its C channels inside the variant; element q * 64 + l of the wave's C * 64
+            # outputs is stored by lane l, so a store instruction writes whole lines
+            W = min(self.sched.tx, 64)
             A(f"{ind}    {{ const unsigned long long vm = __ballot(A.vflag);")
             A(f"{ind}      if (vm) {{")
             A(f"{ind}        const int lane = tid & 63;")
-            A(f"{ind}        const IDX base = ((IDX){z} * N1 + y) * N2 + (x - lane);")
+            A(f"{ind}        const int xw = x - lane % {W}, yw = y - lane / {W};")
             A(f"{ind}        wave_sync();")
             for oi in self.stage:
                 C = self.info[self.group.outputs[oi]].ch
                 A(f"{ind}        {{ const float* st = &so_stage[tid >> 6][{self.stage_off[oi]}];")
                 A(f"{ind}          #pragma unroll")
                 A(f"{ind}          for (int q = 0; q < {C}; ++q) {{")
-                A(f"{ind}            const int e = q * 64 + lane;")
-                A(f"{ind}            if ((vm >> (e / {C})) & 1ull) st_f32((float*)p.out{oi}, base * {C} + e, st[e]);")
+                A(f"{ind}            const int e = q * 64 + lane, v = e / {C};")
+                A(f"{ind}            if ((vm >> v) & 1ull) {{")
+                A(f"{ind}              const IDX o = ((IDX){z} * N1 + yw + v / {W}) * N2 + xw + v % {W};")
+                A(f"{ind}              st_f32((float*)p.out{oi}, o * {C} + (e - v * {C}), st[e]);")
+                A(f"{ind}            }}")
                 A(f"{ind}          }} }}")
             A(f"{ind}        wave_sync();")
             A(f"{ind}      }}")
@@ -1893,6 +1898,24 @@ class FusedExecutor:
                 return s
         return heuristic_schedule(group.grid)
 
+    @staticmethod
+    def _fit_registers(gen: "KernelGen", src: str) -> str:
+        """Keep the main kernel at >= 4 waves per SIMD (<= MERGED_TAIL_MAX_VGPRS registers) when
+        one of two code-size choices pushes it past that: first the LDS staging of wide outputs
+        is dropped (direct stores), then the edge band moves to its own side-stream kernel.
+        Register counts come from the code object's metadata, so no GPU is needed."""
+        can_split = SPLIT_TAIL == "auto" and gen.use_lds and not gen.batched and gen.boundary_blocks() > 0
+        if not (gen.stage or can_split):
+            return src
+        regs = lambda s_: kernel_registers(compile_source(s_)[0]).get("fused_kernel", 0)
+        if gen.stage and regs(src) > MERGED_TAIL_MAX_VGPRS:
+            gen.stage, gen.stage_off, gen.stage_c = [], {}, 0
+            src = gen.source()
+        if can_split and regs(src) > MERGED_TAIL_MAX_VGPRS:
+            gen.want_split = True
+            src = gen.source()
+        return src
+
     def prepare(self, graph: Graph, values: Dict[str, Any], wanted: Sequence[str],
                 out_dtype: Optional[torch.dtype] = None, schedule: Optional[Schedule] = None):
         shapes = tuple(sorted((k, tuple(v.shape) if isinstance(v, torch.Tensor) else None)
@@ -1927,13 +1950,7 @@ class FusedExecutor:
             if gen.lds_bytes() > LDS_LIMIT:
                 raise RuntimeError(f"fused group needs {gen.lds_bytes()} B of LDS (limit {LDS_LIMIT})")
             src = gen.source()
-            if SPLIT_TAIL == "auto" and gen.use_lds and not gen.batched and gen.boundary_blocks() > 0:
-                # in-grid tail blocks beat the side-stream kernel unless their code costs the
-                # main kernel occupancy
-                code, _ = compile_source(src)
-                if kernel_registers(code).get("fused_kernel", 0) > MERGED_TAIL_MAX_VGPRS:
-                    gen.want_split = True
-                    src = gen.source()
+            src = self._fit_registers(gen, src)
             compiled[si] = CompiledGroup(gen, src)
         plan = (steps, info, compiled)
         self._plans[key] = plan

@@ -221,6 +221,13 @@ def autotune(graph, runtime, wanted: Sequence[str], seed: Optional[Dict[str, Any
         gens = candidates_for(graph, st.group, info, base.gen.out_dtypes, group_cands)
         srcs = [g.source() for g in gens]
         codes = _compile_all(srcs, workers)
+        # the executor's register rule (staging and tail placement), so that what is timed is
+        # what would run
+        for j, (gen, code) in enumerate(zip(gens, codes)):
+            if F.kernel_registers(code).get("fused_kernel", 0) > F.MERGED_TAIL_MAX_VGPRS:
+                src = F.FusedExecutor._fit_registers(gen, srcs[j])
+                if src != srcs[j]:
+                    srcs[j], codes[j] = src, F.compile_source(src)[0]
         stats = group_stats(base.gen)
         sig = group_signature(st.group, graph)
         best = None

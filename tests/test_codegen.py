@@ -116,9 +116,12 @@ def test_load_transforms_detected_only_where_they_pay():
 
 
 @pytest.mark.parametrize("sched,staged", [((128, 8, 64, 256), True), ((64, 4, 64, 256), True),
+                                          ((32, 8, 64, 128), True), ((16, 8, 64, 128), True),
                                           ((32, 8, 64, 128), False)])
-def test_wide_outputs_staged_through_lds_only_on_64_wide_tiles(sched, staged):
-    # 9-channel hessian and 3-channel gradient: only outputs of >= 4 channels are staged
+def test_wide_outputs_staged_through_lds(sched, staged, monkeypatch):
+    # 9-channel hessian and 3-channel gradient: only outputs of >= 4 channels are staged; a
+    # wave of a tile narrower than 64 covers 64 / TX consecutive rows
+    monkeypatch.setattr(F, "STAGE_STORES", staged)
     f = torch.zeros((16, 24, 200))
     g, hes, *_ = sp.compile_expression("hessian(f)", boundary="periodic", field_map={"f": f})
     grad = g.add("gradient", (g.node(hes).input_ids[0],), dict(g.node(hes).params))

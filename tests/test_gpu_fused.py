@@ -87,10 +87,11 @@ def test_hessian_tensor_ops(edge_mode):
 
 
 @pytest.mark.parametrize("edge_mode", ["band", "variants"])
-@pytest.mark.parametrize("sched", [Schedule(128, 8, 16, 256), Schedule(64, 4, 16, 256)])
+@pytest.mark.parametrize("sched", [Schedule(128, 8, 16, 256), Schedule(64, 4, 16, 256), Schedule(32, 8, 16, 128),
+                                   Schedule(16, 8, 16, 128)])
 def test_wide_outputs_staged_stores(sched, edge_mode):
-    # 9-channel outputs on 64-wide tiles go through the per-wave LDS transpose; x = 200 leaves
-    # a partial tile whose lanes past the edge store nothing
+    # 9-channel outputs go through the per-wave LDS transpose (a wave spans 64 / TX rows of
+    # narrow tiles); x = 200 and y = 20 leave partial tiles whose lanes past the edge store nothing
     shape = (12, 20, 200)
     f = field(shape, 8)
     for bc in ("periodic", "neumann"):
