@@ -1706,21 +1706,23 @@ class KernelGen:
             # deposited its C channels inside the variant; element q * 64 + l of the wave's C * 64
             # outputs is stored by lane l, so a store instruction writes whole lines
             W = min(self.sched.tx, 64)
+            # element e of the area (voxel v = e / C) lives at C * (lane 0's voxel) + e, plus one
+            # row, (N2 - W) * C further, for every W voxels
             A(f"{ind}    {{ const unsigned long long vm = __ballot(A.vflag);")
             A(f"{ind}      if (vm) {{")
             A(f"{ind}        const int lane = tid & 63;")
-            A(f"{ind}        const int xw = x - lane % {W}, yw = y - lane / {W};")
+            A(f"{ind}        const unsigned vlo = (unsigned)vm, vhi = (unsigned)(vm >> 32);")
+            A(f"{ind}        const IDX v0 = ((IDX){z} * N1 + (y - lane / {W})) * N2 + (x - lane % {W});")
             A(f"{ind}        wave_sync();")
             for oi in self.stage:
                 C = self.info[self.group.outputs[oi]].ch
                 A(f"{ind}        {{ const float* st = &so_stage[tid >> 6][{self.stage_off[oi]}];")
+                A(f"{ind}          float* ob = (float*)p.out{oi} + v0 * {C};")
                 A(f"{ind}          #pragma unroll")
                 A(f"{ind}          for (int q = 0; q < {C}; ++q) {{")
                 A(f"{ind}            const int e = q * 64 + lane, v = e / {C};")
-                A(f"{ind}            if ((vm >> v) & 1ull) {{")
-                A(f"{ind}              const IDX o = ((IDX){z} * N1 + yw + v / {W}) * N2 + xw + v % {W};")
-                A(f"{ind}              st_f32((float*)p.out{oi}, o * {C} + (e - v * {C}), st[e]);")
-                A(f"{ind}            }}")
+                # the ballot's halves as 32-bit words: one register for the test, not two
+                A(f"{ind}            if (((v < 32 ? vlo : vhi) >> (v & 31)) & 1u) st_f32(ob, e + (v / {W}) * ((N2 - {W}) * {C}), st[e]);")
                 A(f"{ind}          }} }}")
             A(f"{ind}        wave_sync();")
             A(f"{ind}      }}")
