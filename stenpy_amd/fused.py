@@ -1903,18 +1903,18 @@ class FusedExecutor:
     @staticmethod
     def _fit_registers(gen: "KernelGen", src: str) -> str:
         """Keep the main kernel at >= 4 waves per SIMD (<= MERGED_TAIL_MAX_VGPRS registers) when
-        one of two code-size choices pushes it past that: first the LDS staging of wide outputs
-        is dropped (direct stores), then the edge band moves to its own side-stream kernel.
+        one of two code-size choices pushes it past that: first the edge band moves to its own
+        side-stream kernel, then the LDS staging of wide outputs is dropped (direct stores).
         Register counts come from the code object's metadata, so no GPU is needed."""
         can_split = SPLIT_TAIL == "auto" and gen.use_lds and not gen.batched and gen.boundary_blocks() > 0
         if not (gen.stage or can_split):
             return src
         regs = lambda s_: kernel_registers(compile_source(s_)[0]).get("fused_kernel", 0)
-        if gen.stage and regs(src) > MERGED_TAIL_MAX_VGPRS:
-            gen.stage, gen.stage_off, gen.stage_c = [], {}, 0
-            src = gen.source()
         if can_split and regs(src) > MERGED_TAIL_MAX_VGPRS:
             gen.want_split = True
+            src = gen.source()
+        if gen.stage and regs(src) > MERGED_TAIL_MAX_VGPRS:
+            gen.stage, gen.stage_off, gen.stage_c = [], {}, 0
             src = gen.source()
         return src
 
