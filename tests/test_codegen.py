@@ -149,3 +149,30 @@ def test_tail_blocks_join_the_main_grid_unless_registers_grow(expr):
     assert gen.split_tail == (regs["fused_kernel"] > F.MERGED_TAIL_MAX_VGPRS)
     if gen.split_tail:
         assert F.kernel_registers(F.compile_source(comp[0].source)[0])["fused_tail"] > 0
+
+
+def test_register_rule_splits_the_tail_before_dropping_staging():
+    # the five-sink tensor group (band edge mode): staged and merged it needs > 128 VGPRs; the
+    # plan first moves the edge band to its own kernel, then stores directly, and ends at <= 128
+    f = torch.zeros((96, 96, 160))
+    exprs = ["eigenvalues(hessian(f))", "determinant(hessian(f))", "trace(hessian(f))",
+             "deviatoric(hessian(f))", "mean_curvature(f)"]
+    from stenpy_amd.compiler import compile_expressions
+    g, sinks, *_ = compile_expressions(exprs, boundary="periodic", field_map={"f": f})
+    values = {n: g.node(n).params["value"] for n in g._order if g.node(n).op_name == "_constant"}
+    steps, info = F.plan_graph(g, values, sinks)
+    gen = F.KernelGen(g, steps[0].group, info, F.Schedule(32, 8, 16, 128), {}, {})
+    assert gen.stage, "the 9-channel deviatoric is eligible for staging"
+    regs = lambda s: F.kernel_registers(F.compile_source(s)[0])["fused_kernel"]
+    before = regs(gen.source())
+    src = F.FusedExecutor._fit_registers(gen, gen.source())
+    after = regs(src)
+    assert before > F.MERGED_TAIL_MAX_VGPRS >= after
+    assert gen.split_tail and "fused_tail" in F.kernel_registers(F.compile_source(src)[0])
+    # a light staged group keeps both its staging and its in-grid tail
+    g2, s2, *_ = compile_expressions(["hessian(f)"], boundary="periodic", field_map={"f": f})
+    values = {n: g2.node(n).params["value"] for n in g2._order if g2.node(n).op_name == "_constant"}
+    steps2, info2 = F.plan_graph(g2, values, s2)
+    gen2 = F.KernelGen(g2, steps2[0].group, info2, F.Schedule(32, 8, 16, 128), {}, {})
+    F.FusedExecutor._fit_registers(gen2, gen2.source())
+    assert gen2.stage and not gen2.split_tail
