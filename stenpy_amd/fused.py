@@ -627,17 +627,20 @@ class _Emit:
     def _unary(op: str, x: str, p: Dict) -> str:
         if op == "neg":
             return f"-{x}"
-        # transcendentals are evaluated in FP64 and rounded once (FP32 library versions are
-        # 1-2 ulp, the reference CPU path is near correctly rounded)
+        # transcendentals round an FP64 value once (FP32 library versions are 1-2 ulp, the
+        # reference CPU path is near correctly rounded): exp, tanh and log through sp_math32.h's
+        # short FP64 evaluations (0.5 ulp + 2^-18), sin and cos through the FP64 library
         if op == "exp":
-            return f"(float)exp((double){x})"
+            return f"spm_expf({x})"
         if op == "log":
             e = _flt(p.get("eps", 1e-15))
-            return f"(float)log((double)({x} < {e} ? {e} : {x}))"
+            return f"spm_logf({x} < {e} ? {e} : {x})"
+        if op == "tanh":
+            return f"spm_tanhf({x})"
         if op == "sqrt":
             e = _flt(p.get("eps", 0.0))
             return f"sqrtf({x} < {e} ? {e} : {x})"
-        if op in ("sin", "cos", "tanh"):
+        if op in ("sin", "cos"):
             return f"(float){op}((double){x})"
         if op == "clamp":
             lo, hi = _flt(p.get("lo", 0.0)), _flt(p.get("hi", 1.0))
@@ -1333,7 +1336,7 @@ class KernelGen:
 
     @staticmethod
     def _header_sources() -> List[str]:
-        return [(CSRC / hdr).read_text().replace("#pragma once", "") for hdr in ("sp_sym3.h", "fused_template.h")]
+        return [(CSRC / hdr).read_text().replace("#pragma once", "") for hdr in ("sp_sym3.h", "sp_math32.h", "fused_template.h")]
 
     def section(self, ns: str, idx64: bool = False) -> str:
         """The group's code inside ``namespace ns`` for a batch module: grid sizes and interior

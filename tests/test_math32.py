@@ -1,0 +1,118 @@
+"""The FP32 exp / tanh / log of the generated kernels (stenpy_amd/csrc/sp_math32.h), built as
+plain C++ for the host and checked on the CPU against FP64 numpy over a strided sweep of every
+FP32 bit pattern plus the special and threshold values.  The bound is the one the header
+states: 0.5 ulp of the FP32 result plus 2^-16 ulp.  The tables are checked entry by entry
+against 60-digit decimal values."""
+import ctypes
+import re
+import shutil
+import subprocess
+from decimal import Decimal, getcontext
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+HDR = ROOT / "stenpy_amd" / "csrc" / "sp_math32.h"
+
+_SHIM = r"""
+#include "sp_math32.h"
+extern "C" void run(int fn, const float* x, float* y, long n) {
+    for (long i = 0; i < n; ++i)
+        y[i] = fn == 0 ? spm_expf(x[i]) : (fn == 1 ? spm_tanhf(x[i]) : spm_logf(x[i]));
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def math32(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("no host C++ compiler")
+    d = tmp_path_factory.mktemp("math32")
+    (d / "shim.cpp").write_text(_SHIM)
+    so = d / "libmath32.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", f"-I{HDR.parent}",
+                    "-o", str(so), str(d / "shim.cpp")], check=True)
+    lib = ctypes.CDLL(str(so))
+    lib.run.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
+
+    def call(fn, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty_like(x)
+        lib.run(fn, x.ctypes.data, y.ctypes.data, x.size)
+        return y
+    return call
+
+
+def _inputs(lo_bits=0, stride=1021):
+    bits = np.arange(lo_bits, 1 << 32, stride, dtype=np.uint64).astype(np.uint32)
+    x = bits.view(np.float32)
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, -1.0, 1e-45, -1e-45, 1.1754942e-38,
+                        3.4028235e38, -3.4028235e38, 88.72283, 88.72284, -87.33654, -103.97208, -103.97209,
+                        9.01, 9.02, -9.02, 1e-8, -1e-8, 0.5, 0.75, 1.5, 0.99999994, 1.0000001],
+                       dtype=np.float32)
+    return np.concatenate([x, special])
+
+
+def _ulp_errors(got, ref64):
+    """|got - ref| in units of the last place of the FP32 rounding of ref (finite refs only)."""
+    with np.errstate(over="ignore", invalid="ignore"):
+        r32 = ref64.astype(np.float32)
+        fin = np.isfinite(r32) & np.isfinite(ref64)
+        _, e = np.frexp(ref64[fin])
+        ulp = np.ldexp(1.0, np.maximum(e - 24, -149))
+        err = np.abs(got[fin].astype(np.float64) - ref64[fin]) / ulp
+    return fin, err
+
+
+@pytest.mark.parametrize("fn,name", [(0, "exp"), (1, "tanh"), (2, "log")])
+def test_accuracy_over_fp32_inputs(math32, fn, name):
+    x = _inputs(lo_bits=fn * 7)
+    got = math32(fn, x)
+    xd = x.astype(np.float64)
+    with np.errstate(all="ignore"):
+        ref = {0: np.exp, 1: np.tanh, 2: np.log}[fn](xd)
+    fin, err = _ulp_errors(got, ref)
+    assert err.max() <= 0.5 + 2.0 ** -16, f"{name}: max error {err.max():.9f} ulp"
+    # non-finite FP32 results: the same inf / NaN as the FP32 rounding of the FP64 value
+    with np.errstate(over="ignore", invalid="ignore"):
+        r32 = ref.astype(np.float32)
+    nf = ~fin
+    assert np.array_equal(got[nf], r32[nf], equal_nan=True), name
+    # mostly correctly rounded
+    assert np.mean(got[fin] != r32[fin]) < 1e-5, name
+
+
+def test_edge_values(math32):
+    assert math32(0, [100.0, -200.0, np.inf, -np.inf]).tolist() == [np.inf, 0.0, np.inf, 0.0]
+    assert np.isnan(math32(0, [np.nan])[0]) and np.isnan(math32(1, [np.nan])[0])
+    assert math32(1, [20.0, -20.0, np.inf, -np.inf, 1e-30]).tolist() == [1.0, -1.0, 1.0, -1.0, np.float32(1e-30)]
+    assert np.signbit(math32(1, [-0.0])[0])
+    lg = math32(2, [0.0, -0.0, -1.0, np.inf, 1.0, np.nan])
+    assert lg[0] == -np.inf and lg[1] == -np.inf and np.isnan(lg[2]) and lg[3] == np.inf and lg[4] == 0.0
+    assert np.isnan(lg[5])
+
+
+def _c_table(name):
+    m = re.search(rf"{name}\[\d+\] = \{{(.*?)\}};", HDR.read_text(), flags=re.S)
+    return [float.fromhex(v.strip().rstrip("f")) for v in m.group(1).split(",") if v.strip()]
+
+
+def _nearest_double(d):
+    f = float(d)
+    cands = [np.nextafter(f, -np.inf), f, np.nextafter(f, np.inf)]
+    return float(min(cands, key=lambda v: abs(Decimal(float(v)) - d)))
+
+
+def test_tables_are_correctly_rounded():
+    getcontext().prec = 60
+    ln2 = Decimal(2).ln()
+    assert _c_table("spm_exp2_64") == [_nearest_double((Decimal(j) / 64 * ln2).exp()) for j in range(64)]
+    c = _c_table("spm_log_c")
+    assert c == [float(np.float32(1.0 / (1.0 + j / 128.0))) for j in range(-32, 65)]
+    assert _c_table("spm_log_t") == [_nearest_double(-Decimal(v).ln()) for v in c]
+    text = HDR.read_text()
+    for value, exact in (("0x1.62e42fefa39efp-7", ln2 / 64), ("0x1.71547652b82fep+6", 64 / ln2),
+                         ("0x1.62e42fefa39efp-1", ln2)):
+        assert value in text and float.fromhex(value) == _nearest_double(exact)
