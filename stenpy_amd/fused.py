@@ -629,7 +629,7 @@ class _Emit:
             return f"-{x}"
         # transcendentals round an FP64 value once (FP32 library versions are 1-2 ulp, the
         # reference CPU path is near correctly rounded): exp, tanh and log through sp_math32.h's
-        # short FP64 evaluations (0.5 ulp + 2^-18), sin and cos through the FP64 library
+        # short FP64 evaluations (0.5 ulp + 2^-16), sin and cos through the FP64 library
         if op == "exp":
             return f"spm_expf({x})"
         if op == "log":
