@@ -50,8 +50,10 @@ _ALIAS: Dict[str, str] = {
     "distance_transform": "distance_transform",
 }
 
-# Ops that take no {dx, boundary} params (stenpy_engine.py:3476-3484), plus the two
-# point-wise extensions of this backend.
+# Ops that take no {dx, boundary} params (stenpy_engine.py:3476-3484), plus the point-wise
+# extensions of this backend and three ops that set omits although their functions take no
+# dx or boundary (cos, fft_complex, ifft_complex: called from an expression there, they raise
+# TypeError on the unexpected keywords).
 _NO_STENCIL_OPS: Set[str] = {
     "add", "sub", "mul", "div", "neg", "clamp", "exp", "log", "sqrt", "sin", "tanh",
     "sum", "mean", "norm_l2", "min_max", "variance", "entropy", "trace", "determinant",
@@ -60,7 +62,7 @@ _NO_STENCIL_OPS: Set[str] = {
     "select_component", "norm_last", "div_last", "neg_stack3", "scale_eye",
     "velocity_dot_grad", "integrate", "cumulative_integral", "spectral_gradient",
     "spectral_laplacian",
-    "dot", "sqnorm", "cast_bf16",
+    "dot", "sqnorm", "cast_bf16", "cos", "fft_complex", "ifft_complex",
 }
 
 _SIMPLIFY_RULES: Dict[Tuple[str, str], str] = {("divergence", "gradient"): "laplacian"}

@@ -116,6 +116,25 @@ def test_aliases_and_registered_names():
         assert g.node(sink).op_name == op, expr
 
 
+def test_every_registered_op_accepts_the_params_the_compiler_gives_it():
+    """Ops outside the no-stencil set receive dx and boundary; each must accept them, so that
+    every registered name is callable from an expression (cos, fft_complex and ifft_complex
+    included, which the reference's set omits)."""
+    import inspect
+    from stenpy_amd.compiler import _NO_STENCIL_OPS
+    from stenpy_amd.registry import OP_REGISTRY
+    bad = []
+    for name, fn in OP_REGISTRY.items():
+        if name in _NO_STENCIL_OPS or name == "_constant":
+            continue
+        ps = inspect.signature(fn).parameters
+        if not (("dx" in ps and "boundary" in ps) or any(p.kind == p.VAR_KEYWORD for p in ps.values())):
+            bad.append(name)
+    assert not bad, bad
+    g, sink, *_ = sp.compile_expression("cos(f) + fft_complex(f)", boundary="periodic", field_map=fmap())
+    assert all(not n.params for n in g.nodes.values() if n.op_name in ("cos", "fft_complex"))
+
+
 def test_errors():
     with pytest.raises(ValueError, match="Unknown operator"):
         sp.compile_expression("nosuchop(f)", field_map=fmap())

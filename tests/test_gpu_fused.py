@@ -42,6 +42,21 @@ def test_single_op_expressions():
         run_and_check(expr, {"f": f})
 
 
+@pytest.mark.parametrize("expr", [
+    "exp(f*110 - 15)",        # down into FP32 subnormals and zero, up to 2^116
+    "exp(f*0.0001)",
+    "tanh(f*12)",             # saturates to +-1 beyond 9.02
+    "tanh(f*0.00001)",
+    "log(f*f)", "log(f)",     # eps clamp of the negative half
+    "log(f*0.001 + 0.002)",
+    "sin(f*1000)", "cos(f*3)", "cos(f*0.001)",
+    "sin(f*700000)",          # beyond the short argument reduction
+])
+def test_transcendentals_over_wide_ranges(expr):
+    """The kernels' exp / tanh / log / sin / cos (field |f| <= 0.87) against the oracle."""
+    run_and_check(expr, {"f": field((20, 24, 66), 7)})
+
+
 @pytest.mark.parametrize("edge_mode", ["band", "variants"])
 @pytest.mark.parametrize("shape", [(32, 32, 32), (21, 37, 70), (9, 8, 130)])
 def test_stencil_graph(shape, edge_mode):
