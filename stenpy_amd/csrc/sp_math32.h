@@ -1,4 +1,5 @@
-// FP32 exp, tanh and log for the run-time generated kernels, accurate to 0.5 ulp + 2^-16 ulp:
+// FP32 exp, tanh, log, sin and cos for the run-time generated kernels, accurate to 0.5 ulp +
+// 2^-16 ulp:
 // each result is the FP32 rounding of an FP64 value whose relative error is below 2^-45, so it
 // is the correctly rounded result except where the exact value lies that close to a rounding
 // midpoint.  They replace FP32 rounding of the FP64 library functions, which gives the same
@@ -7,13 +8,19 @@
 // kernel that keeps global loads in flight makes each call wait for all of them.
 //
 //   exp(x)   x = k ln2 + r, |r| <= ln2/2, exp(x) = 2^k (1 + p(r)), p ~ expm1 of degree 10
-//            (weighted least squares on Chebyshev nodes, relative error 2^-49).
+//            (weighted least squares on Chebyshev nodes, relative error 2^-49; the other fits
+//            below are made the same way).
 //   tanh(x)  |tanh x| = E / (E + 2), E = expm1(2|x|) = (2^k - 1) + 2^k p(r), which keeps the
 //            relative accuracy of E for small |x|; 1 / (E + 2) is the FP32 reciprocal refined
 //            by one Newton step in FP64 (relative error 2^-46).
 //   log(x)   x = 2^e m, m in [sqrt(1/2), sqrt(2)), f = m - 1 (exact), s = f / (2 + f) (same
 //            reciprocal), log x = e ln2 + 2 atanh(s) with 2 atanh(s) = 2s + s^3 q(s^2), q of
 //            degree 5 (relative error 2^-52).
+//   sin(x), cos(x)  for |x| < 2^19: x = k pi/2 + r, |r| <= pi/4, with pi/2 in two parts (the
+//            remainder's error is below 2^-89 absolute), then sin r = r + r^3 S(r^2) and
+//            cos r = 1 - r^2/2 + r^4 C(r^2), S and C of degree 4 (errors 2^-47 relative and 2^-53),
+//            picked and signed by k mod 4.  Larger arguments, infinities and NaN go to the FP64
+//            library functions.
 //
 // Pasted verbatim into every hipRTC source.  The same text builds as plain C++ for the host,
 // where the FP32 reciprocal is a division, so the functions can be checked over all FP32
@@ -24,6 +31,7 @@
 #define SPM_FN __device__ __forceinline__
 #define spm_rcp(v) __builtin_amdgcn_rcpf(v)
 #else
+#include <math.h>
 #define SPM_FN static inline
 #define spm_rcp(v) (1.0f / (v))
 #endif
@@ -103,3 +111,26 @@ SPM_FN float spm_logf(float xf) {
     // zero, negative, infinite and NaN inputs
     return xf > 0.0f ? (xf == __builtin_inff() ? xf : v) : (xf == 0.0f ? -__builtin_inff() : __builtin_nanf(""));
 }
+
+// sin (c = 0) or cos (c = 1)
+SPM_FN float spm_trigf(float xf, int c) {
+    const double x = (double)xf;
+    if (!(__builtin_fabs(x) < 0x1p19)) return (float)(c ? cos(x) : sin(x));
+    const double t = __builtin_fma(x, 0x1.45f306dc9c883p-1, 0x1.8p52);
+    const int k = (int)(unsigned)__builtin_bit_cast(unsigned long long, t) + c;
+    const double kd = t - 0x1.8p52;
+    const double r = __builtin_fma(kd, -0x1.1a62633145c07p-54, __builtin_fma(kd, -0x1.921fb54442d18p+0, x));
+    const double z = r * r;
+    double ps = __builtin_fma(z, -0x1.a963efc6db6e5p-26, 0x1.71d77395617ffp-19);
+    ps = __builtin_fma(ps, z, -0x1.a019f9457a92ap-13);
+    ps = __builtin_fma(ps, z, 0x1.1111110c8f29fp-7);
+    ps = __builtin_fma(ps, z, -0x1.5555555552280p-3);
+    double pc = __builtin_fma(z, 0x1.1bc3bffbe4db7p-29, -0x1.27e01c91eb775p-22);
+    pc = __builtin_fma(pc, z, 0x1.a019fa875674fp-16);
+    pc = __builtin_fma(pc, z, -0x1.6c16c167466abp-10);
+    pc = __builtin_fma(pc, z, 0x1.5555555552f55p-5);
+    const double v = (k & 1) ? __builtin_fma(z * z, pc, __builtin_fma(z, -0.5, 1.0)) : r * __builtin_fma(z, ps, 1.0);
+    return (float)((k & 2) ? -v : v);
+}
+SPM_FN float spm_sinf(float x) { return spm_trigf(x, 0); }
+SPM_FN float spm_cosf(float x) { return spm_trigf(x, 1); }

@@ -628,8 +628,8 @@ class _Emit:
         if op == "neg":
             return f"-{x}"
         # transcendentals round an FP64 value once (FP32 library versions are 1-2 ulp, the
-        # reference CPU path is near correctly rounded): exp, tanh and log through sp_math32.h's
-        # short FP64 evaluations (0.5 ulp + 2^-16), sin and cos through the FP64 library
+        # reference CPU path is near correctly rounded), through sp_math32.h's short FP64
+        # evaluations (0.5 ulp + 2^-16)
         if op == "exp":
             return f"spm_expf({x})"
         if op == "log":
@@ -641,7 +641,7 @@ class _Emit:
             e = _flt(p.get("eps", 0.0))
             return f"sqrtf({x} < {e} ? {e} : {x})"
         if op in ("sin", "cos"):
-            return f"(float){op}((double){x})"
+            return f"spm_{op}f({x})"
         if op == "clamp":
             lo, hi = _flt(p.get("lo", 0.0)), _flt(p.get("hi", 1.0))
             return f"({x} < {lo} ? {lo} : ({x} > {hi} ? {hi} : {x}))"

@@ -133,7 +133,7 @@ def group_stats(gen: F.KernelGen) -> Dict[str, float]:
     return {
         "taps": text.count("lds_s") + text.count("p.src"),
         "lines": len(e.lines),
-        "transc": sum(text.count(f) for f in ("spm_expf(", "spm_logf(", "spm_tanhf(", "sin(", "cos(", "acos(")),
+        "transc": sum(text.count(f) for f in ("spm_expf(", "spm_logf(", "spm_tanhf(", "spm_sinf(", "spm_cosf(", "acos(")),
         "sqrt": text.count("sqrt"),
         "src_ch": src_ch, "out_ch": out_ch, "nred": len(g.reductions),
         "hz": gen.hz, "redge": gen.redge,

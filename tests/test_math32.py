@@ -1,4 +1,4 @@
-"""The FP32 exp / tanh / log of the generated kernels (stenpy_amd/csrc/sp_math32.h), built as
+"""The FP32 exp / tanh / log / sin / cos of the generated kernels (stenpy_amd/csrc/sp_math32.h), built as
 plain C++ for the host and checked on the CPU against FP64 numpy over a strided sweep of every
 FP32 bit pattern plus the special and threshold values.  The bound is the one the header
 states: 0.5 ulp of the FP32 result plus 2^-16 ulp.  The reduction constants are checked against
@@ -20,7 +20,8 @@ _SHIM = r"""
 #include "sp_math32.h"
 extern "C" void run(int fn, const float* x, float* y, long n) {
     for (long i = 0; i < n; ++i)
-        y[i] = fn == 0 ? spm_expf(x[i]) : (fn == 1 ? spm_tanhf(x[i]) : spm_logf(x[i]));
+        y[i] = fn == 0 ? spm_expf(x[i]) : fn == 1 ? spm_tanhf(x[i]) : fn == 2 ? spm_logf(x[i])
+             : fn == 3 ? spm_sinf(x[i]) : spm_cosf(x[i]);
 }
 """
 
@@ -66,13 +67,13 @@ def _ulp_errors(got, ref64):
     return fin, err
 
 
-@pytest.mark.parametrize("fn,name", [(0, "exp"), (1, "tanh"), (2, "log")])
+@pytest.mark.parametrize("fn,name", [(0, "exp"), (1, "tanh"), (2, "log"), (3, "sin"), (4, "cos")])
 def test_accuracy_over_fp32_inputs(math32, fn, name):
     x = _inputs(lo_bits=fn * 7)
     got = math32(fn, x)
     xd = x.astype(np.float64)
     with np.errstate(all="ignore"):
-        ref = {0: np.exp, 1: np.tanh, 2: np.log}[fn](xd)
+        ref = {0: np.exp, 1: np.tanh, 2: np.log, 3: np.sin, 4: np.cos}[fn](xd)
     fin, err = _ulp_errors(got, ref)
     assert err.max() <= 0.5 + 2.0 ** -16, f"{name}: max error {err.max():.9f} ulp"
     # non-finite FP32 results: the same inf / NaN as the FP32 rounding of the FP64 value
@@ -92,6 +93,10 @@ def test_edge_values(math32):
     lg = math32(2, [0.0, -0.0, -1.0, np.inf, 1.0, np.nan])
     assert lg[0] == -np.inf and lg[1] == -np.inf and np.isnan(lg[2]) and lg[3] == np.inf and lg[4] == 0.0
     assert np.isnan(lg[5])
+    tr = math32(3, [0.0, -0.0, np.inf, np.nan, 1e-30, 3e38, 524287.97, 524288.0])
+    assert tr[0] == 0.0 and np.signbit(tr[1]) and np.isnan(tr[2]) and np.isnan(tr[3]) and tr[4] == np.float32(1e-30)
+    assert tr[5] == np.float32(np.sin(np.float64(np.float32(3e38))))
+    assert math32(4, [0.0, np.inf, 1e-30]).tolist()[0::2] == [1.0, 1.0] and np.isnan(math32(4, [np.inf])[0])
 
 
 def _nearest_double(d):
@@ -104,7 +109,11 @@ def test_reduction_constants_are_correctly_rounded():
     getcontext().prec = 60
     ln2 = Decimal(2).ln()
     text = HDR.read_text()
-    for value, exact in (("0x1.62e42fefa39efp-1", ln2), ("0x1.71547652b82fep+0", 1 / ln2)):
+    pi = Decimal("3.14159265358979323846264338327950288419716939937510582097494459")
+    p1 = _nearest_double(pi / 2)
+    for value, exact in (("0x1.62e42fefa39efp-1", ln2), ("0x1.71547652b82fep+0", 1 / ln2),
+                         ("0x1.921fb54442d18p+0", pi / 2), ("0x1.1a62633145c07p-54", pi / 2 - Decimal(p1)),
+                         ("0x1.45f306dc9c883p-1", 2 / pi)):
         assert value in text and float.fromhex(value) == _nearest_double(exact)
     # the integer-rounding shifter and the subnormal scale are exact powers (1.5 2^52, 2^23)
     assert "0x1.8p52" in text and "0x1p23f" in text
