@@ -22,12 +22,12 @@
 //            picked and signed by k mod 4.  Larger arguments, infinities and NaN go to the FP64
 //            library functions.
 //
-// Pasted verbatim into every hipRTC source.  The same text builds as plain C++ for the host,
-// where the FP32 reciprocal is a division, so the functions can be checked over all FP32
-// inputs.
+// Pasted verbatim into every hipRTC source and included by the plain point-wise kernels.  The
+// same text builds as plain C++ for the host, where the FP32 reciprocal is a division, so the
+// functions can be checked over all FP32 inputs.
 #pragma once
 
-#ifdef __HIPCC_RTC__
+#if defined(__HIPCC_RTC__) || defined(__HIPCC__)
 #define SPM_FN __device__ __forceinline__
 #define spm_rcp(v) __builtin_amdgcn_rcpf(v)
 #else

@@ -5,6 +5,7 @@
 // contiguous operands take a 4-wide vector path; anything else (scalar or general
 // broadcasting) takes the strided path.
 #include "sp_common.h"
+#include "sp_math32.h"
 
 using namespace sp;
 
@@ -20,14 +21,14 @@ struct Bcast {
 __device__ __forceinline__ float unary_f(int op, float a, float p0, float p1) {
     switch (op) {
         case SP_NEG: return -a;
-        // transcendentals: FP64 evaluation rounded once to FP32 (the FP32 library versions
-        // are 1-2 ulp; the reference's CPU path is near correctly rounded)
-        case SP_EXP: return (float)exp((double)a);
-        case SP_LOG: return (float)log((double)(a < p0 ? p0 : a));
+        // transcendentals: an FP64 evaluation rounded once to FP32, 0.5 ulp + 2^-16 (the FP32
+        // library versions are 1-2 ulp; the reference's CPU path is near correctly rounded)
+        case SP_EXP: return spm_expf(a);
+        case SP_LOG: return spm_logf(a < p0 ? p0 : a);
         case SP_SQRT: return sqrtf(a < p0 ? p0 : a);
-        case SP_SIN: return (float)sin((double)a);
-        case SP_COS: return (float)cos((double)a);
-        case SP_TANH: return (float)tanh((double)a);
+        case SP_SIN: return spm_sinf(a);
+        case SP_COS: return spm_cosf(a);
+        case SP_TANH: return spm_tanhf(a);
         case SP_CLAMP: return a < p0 ? p0 : (a > p1 ? p1 : a);
         case SP_SCALE: return a * p0;
         case SP_SQUARE: return a * a;
@@ -64,6 +65,26 @@ __global__ void __launch_bounds__(256) k_unary_contig(int op, const float* __res
                                                       TO* __restrict__ out, int64_t n, float p0,
                                                       float p1) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        Store<TO>::put(out, i, unary_f(op, a[i], p0, p1));
+}
+
+// Contiguous unary op on 16-byte aligned input, 4 elements per thread per step.
+template <typename TO>
+__global__ void __launch_bounds__(256) k_unary_vec4(int op, const float* __restrict__ a,
+                                                    TO* __restrict__ out, int64_t n, float p0,
+                                                    float p1) {
+    const int64_t n4 = n / 4;
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float4 x = a4[i];
+        Store<TO>::put(out, 4 * i + 0, unary_f(op, x.x, p0, p1));
+        Store<TO>::put(out, 4 * i + 1, unary_f(op, x.y, p0, p1));
+        Store<TO>::put(out, 4 * i + 2, unary_f(op, x.z, p0, p1));
+        Store<TO>::put(out, 4 * i + 3, unary_f(op, x.w, p0, p1));
+    }
+    for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         Store<TO>::put(out, i, unary_f(op, a[i], p0, p1));
 }
@@ -145,7 +166,10 @@ template <typename TO>
 int unary_impl(int op, const sp_tensor* a, sp_tensor* out, float p0, float p1, hipStream_t st) {
     const int64_t n = numel(out);
     if (n == 0) return SP_OK;
-    if (is_contiguous(a) && numel(a) == n) {
+    if (is_contiguous(a) && numel(a) == n && ((uintptr_t)a->ptr & 15) == 0) {
+        hipLaunchKernelGGL(k_unary_vec4<TO>, dim3(grid_for(n, 4)), dim3(256), 0, st, op,
+                           (const float*)a->ptr, (TO*)out->ptr, n, p0, p1);
+    } else if (is_contiguous(a) && numel(a) == n) {
         hipLaunchKernelGGL(k_unary_contig<TO>, dim3(grid_for(n)), dim3(256), 0, st, op,
                            (const float*)a->ptr, (TO*)out->ptr, n, p0, p1);
     } else {

@@ -29,6 +29,20 @@ def test_unary(op):
     assert_2x(getattr(ops, op)(f), o64, o32, op)
 
 
+@pytest.mark.parametrize("op", ["exp", "tanh", "sin"])
+def test_unary_ragged_and_unaligned(op):
+    """An element count that is not a multiple of 4 (the 4-wide kernel's scalar tail) and an
+    input that starts 4 bytes past an aligned address (the scalar kernel)."""
+    f = field((3, 5, 7), 15)
+    o64, o32 = ref(getattr(O, op), f)
+    assert_2x(getattr(ops, op)(f), o64, o32, op + " ragged")
+    flat = field((4, 6, 9), 16).reshape(-1)
+    g = flat[1:].reshape(5, 43)
+    assert g.is_contiguous() and g.data_ptr() % 16 == 4
+    o64, o32 = ref(getattr(O, op), g)
+    assert_2x(getattr(ops, op)(g), o64, o32, op + " unaligned")
+
+
 def test_clamp_and_scalar_mul():
     f = field(SHAPE, 12)
     o64, o32 = ref(O.clamp, f, lo=-0.2, hi=0.3)
