@@ -4,6 +4,7 @@ FP32 bit pattern plus the special and threshold values.  The bound is the one th
 states: 0.5 ulp of the FP32 result plus 2^-16 ulp.  The reduction constants are checked against
 60-digit decimal values."""
 import ctypes
+import os
 import re
 import shutil
 import subprocess
@@ -117,3 +118,63 @@ def test_reduction_constants_are_correctly_rounded():
         assert value in text and float.fromhex(value) == _nearest_double(exact)
     # the integer-rounding shifter and the subnormal scale are exact powers (1.5 2^52, 2^23)
     assert "0x1.8p52" in text and "0x1p23f" in text
+
+
+_SWEEP = r"""
+#include <math.h>
+#include <string.h>
+#include "sp_math32.h"
+// every FP32 bit pattern: max error in ulps of the FP32 rounding of the x87 long double value,
+// and the count of results that differ from that rounding
+extern "C" void sweep(int fn, double* max_ulp, long long* not_rounded) {
+    double mx = 0.0;
+    long long miss = 0;
+    #pragma omp parallel for reduction(max:mx) reduction(+:miss) schedule(static, 65536)
+    for (long long i = 0; i < (1LL << 32); ++i) {
+        const unsigned u = (unsigned)i;
+        float x;
+        memcpy(&x, &u, 4);
+        float got;
+        long double ref;
+        switch (fn) {
+            case 0: got = spm_expf(x); ref = expl(x); break;
+            case 1: got = spm_tanhf(x); ref = tanhl(x); break;
+            case 2: got = spm_logf(x); ref = logl(x); break;
+            case 3: got = spm_sinf(x); ref = sinl(x); break;
+            default: got = spm_cosf(x); ref = cosl(x); break;
+        }
+        const float r32 = (float)ref;
+        if (isnan(r32) || isinf(r32)) {
+            if (!(got == r32 || (isnan(got) && isnan(r32)))) { ++miss; mx = 1e9; }
+            continue;
+        }
+        int e;
+        frexpl(ref, &e);
+        const long double ulp = ldexpl(1.0L, e - 24 > -149 ? e - 24 : -149);
+        const double err = (double)(fabsl((long double)got - ref) / ulp);
+        if (err > mx) mx = err;
+        if (got != r32) ++miss;
+    }
+    *max_ulp = mx;
+    *not_rounded = miss;
+}
+"""
+
+
+@pytest.mark.skipif(os.environ.get("STENPY_MATH32_EXHAUSTIVE") != "1",
+                    reason="exhaustive sweep (minutes); set STENPY_MATH32_EXHAUSTIVE=1")
+@pytest.mark.parametrize("fn,name", [(0, "exp"), (1, "tanh"), (2, "log"), (3, "sin"), (4, "cos")])
+def test_exhaustive_over_all_fp32_inputs(tmp_path, fn, name):
+    """All 2^32 inputs against x86 80-bit long double (64-bit significand): within 0.5 + 2^-16 ulp,
+    and correctly rounded for all but a handful."""
+    if shutil.which("g++") is None:
+        pytest.skip("no host C++ compiler")
+    (tmp_path / "sweep.cpp").write_text(_SWEEP)
+    so = tmp_path / "libsweep.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
+                    f"-I{HDR.parent}", "-o", str(so), str(tmp_path / "sweep.cpp")], check=True)
+    lib = ctypes.CDLL(str(so))
+    mx, miss = ctypes.c_double(), ctypes.c_longlong()
+    lib.sweep(fn, ctypes.byref(mx), ctypes.byref(miss))
+    assert mx.value <= 0.5 + 2.0 ** -16, f"{name}: max error {mx.value:.9f} ulp"
+    assert miss.value <= 64, f"{name}: {miss.value} results not correctly rounded"
