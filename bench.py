@@ -250,7 +250,8 @@ def bench_stream(args) -> int:
           {"workload": "4608x4096x4096 FP32 seed 7, 256-plane slabs, periodic halo 1; "
                        f"sample = first {st['n_chunks']} slabs", "slabs": st["n_chunks"],
            "host_gen_s": round(st["host_gen_s"], 2), "l2_of_sample": st["l2"],
-           "h2d_gbs": round(vox * 4 / st["wall_s"] / 1e9, 2)})
+           # FP32 input bytes over the wall time: the end-to-end feed rate, not the PCIe link rate
+           "input_gbs_end_to_end": round(vox * 4 / st["wall_s"] / 1e9, 2)})
     return 0
 
 
