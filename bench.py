@@ -259,6 +259,11 @@ def copy_bytes(n: int) -> int:
     return 4 * n ** 3
 
 
+# the profile pass's streaming copy is the only unary point-wise kernel it launches (4-wide
+# k_unary_vec4 on aligned input; older traces name the one-element k_unary_contig)
+CALIBRATION_COPY = "k_unary_"
+
+
 def profile_pass(args) -> int:
     """Fused stencil-graph launches only (plus one calibration copy): what rocprofv3 measures."""
     import torch
@@ -288,10 +293,10 @@ def offline_reports(args) -> int:
     from stenpy_amd import profiling
     if args.pmc_traffic:
         t = profiling.traffic(args.pmc_traffic[0], args.pmc_traffic[1], kernel="fused_kernel",
-                              copy_kernel="k_unary_contig", copy_bytes=copy_bytes(args.n))
+                              copy_kernel=CALIBRATION_COPY, copy_bytes=copy_bytes(args.n))
         try:        # the edge-band tail kernel of the same group, when it runs separately
             tt = profiling.traffic(args.pmc_traffic[0], args.pmc_traffic[1], kernel="fused_tail",
-                                   copy_kernel="k_unary_contig", copy_bytes=copy_bytes(args.n))
+                                   copy_kernel=CALIBRATION_COPY, copy_bytes=copy_bytes(args.n))
             t["tail_bytes"] = tt["bytes_per_launch"]
             t["main_bytes"] = t["bytes_per_launch"]
             t["bytes_per_launch"] = t["main_bytes"] + t["tail_bytes"]
