@@ -176,3 +176,13 @@ def test_register_rule_splits_the_tail_before_dropping_staging():
     gen2 = F.KernelGen(g2, steps2[0].group, info2, F.Schedule(32, 8, 16, 128), {}, {})
     F.FusedExecutor._fit_registers(gen2, gen2.source())
     assert gen2.stage and not gen2.split_tail
+
+
+def test_schedule_features_count_each_transcendental_call():
+    """The schedule model's ``transc`` feature counts the math calls the emitter writes, one per
+    op (a renamed math function would otherwise silently zero it)."""
+    from stenpy_amd import tune
+    for expr, want in [("tanh(f)*exp(f) + laplacian(f)", 2), ("sin(f) + cos(f) + log(f*f)", 3),
+                       ("laplacian(f) * 2", 0)]:
+        g, sink, reds, steps, comp = plan(expr)
+        assert sum(tune.group_stats(c.gen)["transc"] for c in comp.values()) == want, expr
