@@ -204,3 +204,47 @@ def test_compile_expressions_shares_subexpressions_across_sinks():
     for e, s in zip(exprs, sinks):
         g1, s1, *_ = sp.compile_expression(e, boundary="periodic", field_map={"f": f})
         assert g1.node(s1).op_name == g.node(s).op_name and g1.node(s1).params == g.node(s).params
+
+
+def test_random_pointwise_expressions_lower_to_their_values():
+    """Seeded random point-wise expressions: the compiled graph, evaluated by the FP64 oracle,
+    equals the expression evaluated directly by SymPy's numpy printer.  Field values lie in
+    [0.5, 1.5], so the eps clamps of div and log never act."""
+    import random
+    import numpy as np
+    rng = random.Random(11)
+    f = torch.rand(3, 4, 5, dtype=torch.float64) + 0.5
+    from oracle import graph_eval
+
+    def rand_expr(depth):
+        if depth == 0 or rng.random() < 0.25:
+            return rng.choice(["f", f"{rng.uniform(0.5, 2.0):.3f}"])
+        kind = rng.random()
+        if kind < 0.4:
+            op = rng.choice(["exp", "tanh", "sin", "cos", "sqrt", "log"])
+            inner = rand_expr(depth - 1)
+            if op in ("sqrt", "log"):
+                inner = f"({inner})**2 + 0.5"
+            if op == "exp":
+                inner = f"tanh({inner})"
+            return f"{op}({inner})"
+        if kind < 0.55:
+            return f"({rand_expr(depth - 1)})**{rng.choice([2, 3, -1, -2, 0.5])}"
+        a, b = rand_expr(depth - 1), rand_expr(depth - 1)
+        op = rng.choice(["+", "-", "*", "/"])
+        if op == "/":
+            b = f"(({b})**2 + 0.5)"
+        return f"({a}) {op} ({b})"
+
+    checked = 0
+    for _ in range(60):
+        expr = rand_expr(4)
+        if "f" not in expr:
+            continue
+        g, sink, *_ = sp.compile_expression(expr, field_map={"f": f})
+        got = graph_eval.evaluate(g, torch.float64)[sink]
+        fn = sympy.lambdify(sympy.Symbol("f"), sympy.sympify(expr), "numpy")
+        want = np.broadcast_to(np.asarray(fn(f.numpy()), dtype=np.float64), f.shape)
+        np.testing.assert_allclose(got.numpy(), want, rtol=1e-12, atol=1e-12, err_msg=expr)
+        checked += 1
+    assert checked >= 40
