@@ -1,40 +1,61 @@
 """Graph IR: the DAG of operator nodes that every executor consumes.
 
-Behaviour follows the reference IR (stenpy_engine.py:351-451):
-  * node ids are ``"<op>_<n>"`` from one process-wide counter (stenpy_engine.py:88-91);
-  * ``topological_sort`` is Kahn's algorithm and raises ``ValueError`` on an unknown input id
-    or a cycle (stenpy_engine.py:384-407);
-  * ``clone_with_replacement`` swaps one source node for a ``_constant`` holding a chunk
-    tensor and keeps every id (stenpy_engine.py:410-419);
-  * shape/dtype side tables (stenpy_engine.py:428-437) and the consumer map (439-444).
+Public surface kept from the reference IR (stenpy_engine.py:351-451), because drop-in callers
+use it: ``GraphNode(id, op_name, input_ids, params)``; ``Graph.add(op_name, input_ids=(),
+params=None, node_id=None) -> id`` with generated ids of the form ``"<op>_<n>"``
+(stenpy_engine.py:88-91); ``topological_sort()`` raising ``ValueError`` on an unknown input id or
+a cycle; ``clone_with_replacement(source_id, tensor)``; the shape/dtype side table;
+``build_consumer_map()``; ``_graph_sinks``.  Reference-side code also reads ``graph._nodes``,
+``graph._order`` and ``graph._shapes`` directly (fused_compiler.py:237-726,
+stenpy_engine.py:449-457, 961-968), so those names exist here as read-only views.
+
+Design.  Nodes live in one insertion-ordered table, and the reverse edges (who consumes a node)
+are kept up to date as nodes are added, so the consumer map, the sink list and dead-code
+elimination never rescan the graph.  The topological order is a depth-first post-order over the
+insertion order: a graph whose nodes were added after their inputs (everything the compiler
+builds) comes back in insertion order, and any other valid DAG still comes back with every node
+after its inputs.
 
 Additions for the GPU executors: ``sinks()``, ``dead_code_eliminate()`` (the reference executes
 dead nodes, e.g. the gradient left behind by the divergence(gradient) rewrite) and
-``canonical_key()``, a structural hash used to cache fused plans.  The reference's fused
-cache hashes only the set of op names (fused_compiler.py:537-543), which collides; the key
-here covers the whole DAG.
+``canonical_key()``, a structural hash used to cache fused plans.  The reference's fused cache
+hashes only the set of op names (fused_compiler.py:537-543), which collides; the key here covers
+the whole DAG.
 """
 from __future__ import annotations
 
-import collections
 import hashlib
 import itertools
+import threading
 from dataclasses import dataclass, field as dc_field
 from typing import Any, Dict, Iterable, List, Optional, Tuple
 
 import torch
 
-_call_counter = itertools.count()
+
+class _IdSource:
+    """Process-wide source of node ids ``<op>_<n>``; ``n`` never repeats within a process."""
+
+    def __init__(self) -> None:
+        self._next = itertools.count()
+        self._lock = threading.Lock()
+
+    def __call__(self, op_name: str) -> str:
+        with self._lock:
+            return f"{op_name}_{next(self._next)}"
+
+
+_fresh_id = _IdSource()
 
 
 def _uid(prefix: str = "t") -> str:
-    """Process-unique node id ``<prefix>_<n>`` (stenpy_engine.py:88-91)."""
-    return f"{prefix}_{next(_call_counter)}"
+    """A new process-unique id ``<prefix>_<n>``."""
+    return _fresh_id(prefix)
 
 
 @dataclass(frozen=True)
 class GraphNode:
-    """One operator application (stenpy_engine.py:351-356)."""
+    """One operator application: ``op_name(*input_ids, **params)``."""
     id: str
     op_name: str
     input_ids: Tuple[str, ...] = dc_field(default_factory=tuple)
@@ -42,51 +63,88 @@ class GraphNode:
 
 
 class Graph:
-    """Insertion-ordered DAG of :class:`GraphNode` (stenpy_engine.py:359-444)."""
+    """Insertion-ordered DAG of :class:`GraphNode`, with reverse edges kept incrementally."""
 
     def __init__(self) -> None:
-        self._nodes: Dict[str, GraphNode] = {}
-        self._order: List[str] = []
-        self._dtypes: Dict[str, torch.dtype] = {}
-        self._shapes: Dict[str, Tuple[int, ...]] = {}
+        self._table: Dict[str, GraphNode] = {}
+        self._users: Dict[str, List[str]] = {}
+        self._meta: Dict[str, Tuple[Tuple[int, ...], torch.dtype]] = {}
 
     # -- construction ---------------------------------------------------------------------
     def add(self, op_name: str, input_ids: Iterable[str] = (),
             params: Optional[Dict[str, Any]] = None, node_id: Optional[str] = None) -> str:
-        nid = node_id or _uid(op_name)
-        self._nodes[nid] = GraphNode(id=nid, op_name=op_name, input_ids=tuple(input_ids),
-                                     params=dict(params or {}))
-        if nid not in self._order:
-            self._order.append(nid)
-        return nid
+        """Insert a node (or replace the node of the same id in place) and return its id."""
+        node = GraphNode(node_id or _fresh_id(op_name), op_name, tuple(input_ids),
+                         dict(params or {}))
+        old = self._table.get(node.id)
+        if old is not None:
+            for src in old.input_ids:
+                self._users[src].remove(old.id)
+        self._table[node.id] = node
+        for src in node.input_ids:
+            self._users.setdefault(src, []).append(node.id)
+        return node.id
+
+    # -- read-only views kept for reference-side callers ------------------------------------
+    @property
+    def _nodes(self) -> Dict[str, GraphNode]:
+        return self._table
+
+    @property
+    def _order(self) -> List[str]:
+        return list(self._table)
+
+    @property
+    def _shapes(self) -> Dict[str, Tuple[int, ...]]:
+        return {nid: m[0] for nid, m in self._meta.items()}
+
+    @property
+    def _dtypes(self) -> Dict[str, torch.dtype]:
+        return {nid: m[1] for nid, m in self._meta.items()}
+
+    @property
+    def nodes(self) -> Dict[str, GraphNode]:
+        return self._table
+
+    def node(self, nid: str) -> GraphNode:
+        return self._table[nid]
 
     # -- traversal ------------------------------------------------------------------------
     def topological_sort(self) -> List[GraphNode]:
-        in_deg = {nid: 0 for nid in self._nodes}
-        children: Dict[str, List[str]] = {nid: [] for nid in self._nodes}
-        for nid, node in self._nodes.items():
-            for dep in node.input_ids:
-                if dep not in self._nodes:
-                    raise ValueError(f"Node '{nid}' references unknown input '{dep}'")
-                in_deg[nid] += 1
-                children[dep].append(nid)
-        queue = collections.deque(nid for nid, d in in_deg.items() if d == 0)
-        order: List[GraphNode] = []
-        while queue:
-            nid = queue.popleft()
-            order.append(self._nodes[nid])
-            for child in children[nid]:
-                in_deg[child] -= 1
-                if in_deg[child] == 0:
-                    queue.append(child)
-        if len(order) != len(self._nodes):
-            raise ValueError("Graph contains a cycle")
-        return order
+        """Every node after all of its inputs; ``ValueError`` on an unknown input or a cycle."""
+        OPEN, DONE = 1, 2
+        mark: Dict[str, int] = {}
+        out: List[GraphNode] = []
+        for root in self._table:
+            if root in mark:
+                continue
+            mark[root] = OPEN
+            path = [(root, iter(self._table[root].input_ids))]
+            while path:
+                nid, pending = path[-1]
+                src = next(pending, None)
+                if src is None:
+                    path.pop()
+                    mark[nid] = DONE
+                    out.append(self._table[nid])
+                    continue
+                if src not in self._table:
+                    raise ValueError(f"node '{nid}' has an unknown input '{src}'")
+                state = mark.get(src)
+                if state == OPEN:
+                    raise ValueError(f"dependency cycle through node '{src}'")
+                if state is None:
+                    mark[src] = OPEN
+                    path.append((src, iter(self._table[src].input_ids)))
+        return out
 
     def clone_with_replacement(self, source_id: str, tensor: Any) -> "Graph":
+        """A copy in which node ``source_id`` becomes a ``_constant`` holding ``tensor``.
+
+        Every id is kept, so results of the copy are keyed like the original's (the chunk
+        streamer swaps a field for one slab this way)."""
         g = Graph()
-        for nid in self._order:
-            node = self._nodes[nid]
+        for nid, node in self._table.items():
             if nid == source_id:
                 g.add("_constant", (), {"value": tensor}, node_id=nid)
             else:
@@ -94,18 +152,12 @@ class Graph:
         return g
 
     def build_consumer_map(self) -> Dict[str, List[str]]:
-        consumers: Dict[str, List[str]] = {nid: [] for nid in self._nodes}
-        for nid, node in self._nodes.items():
-            for dep in node.input_ids:
-                consumers[dep].append(nid)
-        return consumers
+        """node id -> ids of the nodes that read it, in insertion order."""
+        return {nid: list(self._users.get(nid, ())) for nid in self._table}
 
     def sinks(self) -> List[str]:
         """Nodes nobody consumes, in insertion order (stenpy_engine.py:447-451)."""
-        consumed = set()
-        for node in self._nodes.values():
-            consumed.update(node.input_ids)
-        return [nid for nid in self._order if nid not in consumed]
+        return [nid for nid in self._table if not self._users.get(nid)]
 
     # -- optimisation ---------------------------------------------------------------------
     def dead_code_eliminate(self, keep: Iterable[str]) -> "Graph":
@@ -116,18 +168,16 @@ class Graph:
             nid = stack.pop()
             if nid in live:
                 continue
-            if nid not in self._nodes:
+            if nid not in self._table:
                 raise ValueError(f"unknown node '{nid}'")
             live.add(nid)
-            stack.extend(self._nodes[nid].input_ids)
+            stack.extend(self._table[nid].input_ids)
         g = Graph()
-        for nid in self._order:
+        for nid, n in self._table.items():
             if nid in live:
-                n = self._nodes[nid]
                 g.add(n.op_name, n.input_ids, n.params, node_id=nid)
-                if nid in self._shapes:
-                    g._shapes[nid] = self._shapes[nid]
-                    g._dtypes[nid] = self._dtypes[nid]
+                if nid in self._meta:
+                    g._meta[nid] = self._meta[nid]
         return g
 
     def canonical_key(self, extra: Any = None) -> str:
@@ -151,33 +201,28 @@ class Graph:
             h.update(repr(extra).encode())
         return h.hexdigest()
 
-    # -- side tables (stenpy_engine.py:428-437) -------------------------------------------
+    # -- shape/dtype side table -----------------------------------------------------------
     def set_node_shape(self, node_id: str, shape: Tuple[int, ...],
                        dtype: torch.dtype = torch.float32) -> None:
-        self._shapes[node_id] = tuple(shape)
-        self._dtypes[node_id] = dtype
+        self._meta[node_id] = (tuple(shape), dtype)
 
     def get_node_shape(self, node_id: str) -> Optional[Tuple[int, ...]]:
-        return self._shapes.get(node_id)
+        m = self._meta.get(node_id)
+        return None if m is None else m[0]
 
     def get_node_dtype(self, node_id: str) -> torch.dtype:
-        return self._dtypes.get(node_id, torch.float32)
-
-    def node(self, nid: str) -> GraphNode:
-        return self._nodes[nid]
-
-    @property
-    def nodes(self) -> Dict[str, GraphNode]:
-        return self._nodes
+        """FP32 when unset: this backend stores fields in FP32 (the reference assumes FP64)."""
+        m = self._meta.get(node_id)
+        return torch.float32 if m is None else m[1]
 
     def __len__(self) -> int:
-        return len(self._nodes)
+        return len(self._table)
 
     def __contains__(self, nid: str) -> bool:
-        return nid in self._nodes
+        return nid in self._table
 
     def __repr__(self) -> str:
-        return f"Graph({len(self._nodes)} nodes)"
+        return f"Graph({len(self._table)} nodes)"
 
 
 def _graph_sinks(graph: Graph) -> List[str]:

@@ -499,3 +499,101 @@ def test_statistics_oracle_against_numpy_and_scipy():
     # cumulative trapezoid: its last entry is numpy's trapezoid rule over the axis
     assert np.allclose(O.cumulative_integral(a, dx=0.5, dim=2).numpy()[..., -1],
                        np.trapezoid(a.numpy(), dx=0.5, axis=2), rtol=1e-12)
+
+
+# ------------------------------------------------------------------ channel-last helpers
+# stenpy_engine.py:2199-2313.  Each check is a closed form or a per-voxel loop written from
+# the formula, with distinct per-channel values so that a swapped channel, sign or axis fails.
+def _lin_grid(shape):
+    return [torch.from_numpy(a) for a in np.meshgrid(*[np.arange(n) * 1.0 for n in shape],
+                                                     indexing="ij")]
+
+
+def test_dot_of_gradient_of_linear_field_with_constant_vector():
+    shape, dx = (5, 6, 7), (0.5, 1.0, 2.0)
+    x, y, z = _lin_grid(shape)
+    a, b, c = 3.0, -5.0, 7.0
+    f = a * x * dx[0] + b * y * dx[1] + c * z * dx[2]     # df/dx_d = a, b, c exactly
+    g = O.gradient_nd(f, dx)
+    w = torch.stack([torch.full(shape, v, dtype=D64) for v in (2.0, 11.0, -13.0)], -1)
+    assert torch.equal(O.dot(w, g), torch.full(shape, 2 * a + 11 * b - 13 * c, dtype=D64))
+    assert torch.equal(O.dot(g, w), O.dot(w, g))
+    u, v = rand((3, 4, 5, 3)), rand((3, 4, 5, 3))
+    ex = bf_grid(lambda i: float(u[i][0] * v[i][0] + u[i][1] * v[i][1] + u[i][2] * v[i][2]),
+                 (3, 4, 5))
+    torch.testing.assert_close(O.dot(u, v), ex, rtol=0, atol=1e-14)
+
+
+def test_sqnorm_of_curl_of_linear_field_is_constant():
+    shape = (5, 6, 7)
+    x, y, z = _lin_grid(shape)
+    M = np.array([[1.0, 2.0, -3.0], [4.0, -1.0, 5.0], [-6.0, 7.0, 2.0]])
+    X = (x, y, z)
+    u = torch.stack([sum(M[cc, d] * X[d] for d in range(3)) for cc in range(3)], -1)
+    # curl(Mx) = (M21 - M12, M02 - M20, M10 - M01), exact at one-sided edges too
+    cv = (M[2, 1] - M[1, 2], M[0, 2] - M[2, 0], M[1, 0] - M[0, 1])
+    assert torch.equal(O.curl(u, 1.0, "neumann"),
+                       torch.stack([torch.full(shape, v, dtype=D64) for v in cv], -1))
+    assert torch.equal(O.sqnorm(O.curl(u, 1.0, "neumann")),
+                       torch.full(shape, sum(v * v for v in cv), dtype=D64))
+    r = rand((3, 4, 5, 3))
+    ex = bf_grid(lambda i: float(r[i][0] ** 2 + r[i][1] ** 2 + r[i][2] ** 2), (3, 4, 5))
+    torch.testing.assert_close(O.sqnorm(r), ex, rtol=0, atol=1e-14)
+
+
+def test_surface_normals_of_a_plane_exact_everywhere():
+    shape, dx = (6, 9), (0.5, 2.0)
+    i, j = np.meshgrid(np.arange(shape[0]) * dx[0], np.arange(shape[1]) * dx[1], indexing="ij")
+    a, b = 3.0, -0.25
+    n = O.surface_normals(torch.from_numpy(a * i + b * j), dx)
+    m = math.sqrt(a * a + b * b + 1.0)
+    ex = torch.tensor([-a / m, -b / m, 1.0 / m], dtype=D64).expand(shape + (3,))
+    torch.testing.assert_close(n, ex, rtol=0, atol=1e-15)   # edge rows included
+    with pytest.raises(ValueError):
+        O.surface_normals(torch.zeros(3, 3, 3, dtype=D64))
+
+
+def test_norm_last_and_neg_stack3_brute_force():
+    t = rand((3, 4, 3))
+    t[1, 2] = 0.0                                   # clamp: |0| -> eps
+    out = O.norm_last(t)
+    assert out.shape == (3, 4, 1)
+    ex = bf_grid(lambda i: max(math.sqrt(sum(float(t[i][c]) ** 2 for c in range(3))), 1e-15),
+                 (3, 4))
+    torch.testing.assert_close(out[..., 0], ex, rtol=1e-15, atol=0)
+    assert float(out[1, 2, 0]) == 1e-15
+    gx, gy = rand((3, 4)), rand((3, 4))
+    ns = O.neg_stack3(gx, gy)
+    for i in np.ndindex(3, 4):
+        m = math.sqrt(float(gx[i]) ** 2 + float(gy[i]) ** 2 + 1.0)
+        ex3 = torch.tensor([-float(gx[i]) / m, -float(gy[i]) / m, 1.0 / m], dtype=D64)
+        torch.testing.assert_close(ns[i], ex3, rtol=2e-16, atol=0)
+
+
+def test_div_last_scale_eye_stack_select_brute_force():
+    a, b, b_full = rand((3, 4, 3)), rand((3, 4)), rand((3, 4, 3))
+    q = O.div_last(a, b)                              # b gains a trailing axis
+    for i in np.ndindex(3, 4, 3):
+        assert float(q[i]) == float(a[i]) / float(b[i[:2]])
+    assert torch.equal(O.div_last(a, b_full), a / b_full)
+    s = rand((3, 4))
+    e = O.scale_eye(s, 3)
+    assert e.shape == (3, 4, 3, 3)
+    for i in np.ndindex(3, 4, 3, 3):
+        assert float(e[i]) == (float(s[i[:2]]) if i[2] == i[3] else 0.0)
+    p, q2, r = rand((3, 4)), rand((3, 4)), rand((3, 4))
+    st = O.stack_components(p, q2, r)
+    for i in np.ndindex(3, 4):
+        assert [float(st[i][c]) for c in range(3)] == [float(p[i]), float(q2[i]), float(r[i])]
+    for c, src in enumerate((p, q2, r)):
+        assert torch.equal(O.select_component(st, c), src)
+    assert torch.equal(O.velocity_dot_grad(p, q2), bf_grid(lambda i: float(p[i]) * float(q2[i]), (3, 4)))
+
+
+def test_material_derivative_of_linear_field_with_constant_velocity():
+    shape, dx = (4, 5, 6), (0.5, 1.0, 2.0)
+    x, y, z = _lin_grid(shape)
+    f = 3.0 * x * dx[0] - 2.0 * y * dx[1] + 5.0 * z * dx[2]
+    v = torch.stack([torch.full(shape, w, dtype=D64) for w in (1.5, -4.0, 0.25)], -1)
+    assert torch.equal(O.material_derivative(f, v, dx),
+                       torch.full(shape, 1.5 * 3 + 4.0 * 2 + 0.25 * 5, dtype=D64))
