@@ -80,21 +80,37 @@ def cpu_baseline(n: int, planes: int):
     return vox / ts[1], threads, f"{planes}x{n}x{n} slab, FP64 torch CPU oracle, median of 3"
 
 
-def device_copy_gbs(dev) -> float:
-    """Read+write GB/s of a 2 GiB device-to-device copy on this box (a reference for the
-    HBM rate this particular GPU sustains; it varies between boxes)."""
+def hbm_ceilings(f, u, reps: int = 10):
+    """What this GPU sustains for three read:write mixes, from the library's streaming probes
+    on the benchmark's own buffers (f: n^3 FP32, u: n^3 x 3): copy of f (1:1), the stencil
+    graph's mix (read f and u, write one float per voxel: 16 B read, 4 B written, 4:1) and a
+    read-only pass over u.  HIP events on the launch stream; the best of the flat and the
+    persistent launch shape, each the mean of ``reps`` back-to-back launches.  GB/s."""
     import torch
-    a = torch.empty(1 << 28, device=dev)
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(10):
-        b.copy_(a)
-    e1.record()
-    e1.synchronize()
-    return round(2 * a.numel() * 4 * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+    from stenpy_amd import _lib as L
+    n = f.numel()
+    out = torch.empty_like(f)
+    cases = {"copy": (8 * n, dict(a=f, out=out)), "4r1w": (20 * n, dict(a=f, u=u, out=out)),
+             "read": (12 * n, dict(a=u))}
+    res = {}
+    for mix, (nbytes, kw) in cases.items():
+        best = 0.0
+        for persistent in (False, True):
+            if mix == "read":
+                kw["partials"] = torch.empty(L.probe_blocks(persistent, kw["a"].numel()),
+                                             dtype=torch.float64, device=f.device)
+            for _ in range(3):
+                L.probe_stream(mix, persistent, **kw)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                L.probe_stream(mix, persistent, **kw)
+            e1.record()
+            e1.synchronize()
+            best = max(best, nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+        res[mix] = round(best, 1)
+    del out
+    return res
 
 
 def schedule_source(cg, g) -> str:
@@ -391,7 +407,7 @@ def main() -> int:
     # average duration of the group's kernels over the timed launches (HIP events on their
     # stream), the same launch sequence the kernel-statistics profile records
     k_ms = ln.time_ms(reps=args.steps, warmup=args.warmup, rounds=1)
-    copy_gbs = device_copy_gbs(dev)
+    ceil = hbm_ceilings(f, u)
     alg_bytes = 20 * nvox
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
     traffic = None
@@ -421,11 +437,13 @@ def main() -> int:
                    "step": "hipGraph replay of the fused evaluation (fused kernels, 4 FP64 combines)",
                    "eager_launch_gvoxel_s": round(nvox / t_fused_eager / 1e9, 2),
                    "eager_launch_ms_per_step": round(t_fused_eager * 1e3, 4),
-                   "device_copy_gbs": copy_gbs,
+                   "probe_gbs": {"copy_1r1w": ceil["copy"], "stencil_mix_4r1w": ceil["4r1w"],
+                                 "read_only": ceil["read"]},
                    "bf16_out_hbm_frac_step": round(18 * nvox / t_bf16 / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel_ms": round(k_ms, 4),
+                     "frac_of_probe_4r1w": round(achieved / ceil["4r1w"], 4),
                      "kernels": ("fused_kernel (LDS z-march, face wave) + fused_tail (edge band, side stream)"
                                  if cg.gen.split_tail else
                                  "fused_kernel (LDS z-march, face wave, edge-band blocks at the end of the grid)"),

@@ -265,6 +265,20 @@ int sp_mlp_adam(float* w, const float* g, float* m, float* v, int64_t n, float l
                 float beta2, float eps, int step, float weight_decay, void* w_bf16, void* stream);
 int sp_cast_f32_bf16(const float* x, int64_t n, void* y, void* stream);
 
+/* ---- measurement probes (no reference counterpart) ------------------------------------- */
+/* HBM streaming probes that measure what this GPU sustains for a read:write mix, reported
+ * beside the 8 TB/s roofline.  COPY: out = a (1:1); 4R1W: reads a (n floats) and u (3n floats,
+ * as three coalesced thirds), writes out (n floats), the stencil graph's 16 B read / 4 B written
+ * per voxel;
+ * READ: one double per block (sum of its elements) into ``partials``.  ``persistent`` = 0
+ * launches one float4 group per thread; 1 launches 8 blocks per CU that stride over the range.
+ * n must be a multiple of 4 and every pointer 16-byte aligned. */
+enum { SP_PROBE_COPY = 0, SP_PROBE_4R1W = 1, SP_PROBE_READ = 2 };
+int     sp_probe_stream(int mix, int persistent, const float* a, const float* u, float* out,
+                        double* partials, int64_t n, int n_cu, void* stream);
+/* Blocks (and so ``partials`` entries) that sp_probe_stream launches for these arguments. */
+int64_t sp_probe_blocks(int persistent, int64_t n, int n_cu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -118,6 +118,8 @@ def _declare(lib: C.CDLL) -> None:
         "sp_mlp_adam": ([V, V, V, V, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, I,
                          C.c_float, V, V], I),
         "sp_cast_f32_bf16": ([V, C.c_int64, V, V], I),
+        "sp_probe_stream": ([I, I, V, V, V, V, C.c_int64, I, V], I),
+        "sp_probe_blocks": ([I, C.c_int64, I], C.c_int64),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -438,3 +440,22 @@ def host_gen_fourier(out_ptr: int, shape, z0: int, nz: int, ks: np.ndarray, amps
                                    C.c_uint64(seed & ((1 << 64) - 1)), int(threads))
     if rc != 0:
         raise ValueError("host_gen_fourier: bad arguments")
+
+
+PROBE = {"copy": 0, "4r1w": 1, "read": 2}
+
+
+def probe_stream(mix: str, persistent: bool, a: torch.Tensor, u: Optional[torch.Tensor] = None,
+                 out: Optional[torch.Tensor] = None, partials: Optional[torch.Tensor] = None,
+                 n_cu: int = 256) -> None:
+    """One launch of an HBM streaming probe (``sp_probe_stream``) on torch's current stream."""
+    require_gpu(a)
+    check(lib().sp_probe_stream(PROBE[mix], int(persistent), a.data_ptr(),
+                                u.data_ptr() if u is not None else None,
+                                out.data_ptr() if out is not None else None,
+                                partials.data_ptr() if partials is not None else None,
+                                a.numel(), n_cu, stream_ptr(a.device)), "probe " + mix)
+
+
+def probe_blocks(persistent: bool, n: int, n_cu: int = 256) -> int:
+    return int(lib().sp_probe_blocks(int(persistent), n, n_cu))

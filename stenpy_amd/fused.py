@@ -157,6 +157,18 @@ def _scalar_value(v: Any) -> Optional[float]:
     return None
 
 
+def _value_signature(v: Any) -> Tuple:
+    """What a plan depends on in a seeded or materialised value: a host scalar is compiled into
+    the kernel as a literal, so its value is part of the plan; a device scalar is read at run
+    time and a field only contributes its shape and dtype."""
+    s = _scalar_value(v)
+    if s is not None:
+        return ("const", s)
+    if isinstance(v, torch.Tensor):
+        return ("tensor", tuple(v.shape), str(v.dtype), v.is_cuda)
+    return ("other", type(v).__name__)
+
+
 def _dx3(dx: Any) -> Tuple[float, float, float]:
     if isinstance(dx, (int, float)):
         return (float(dx),) * 3
@@ -1215,6 +1227,39 @@ class KernelGen:
         A("        return;")
         A("    }")
 
+    def accumulators(self) -> Tuple[List[Tuple[str, str, Optional[int]]], Dict[int, List[int]]]:
+        """Per-thread accumulators shared between the group's reductions, and which ones each
+        reduction reads.  Kinds: "s1" compensated sum of x (sum, mean, variance), "s2"
+        compensated sum of x^2 with the exact FMA split of each square (variance, norm_l2),
+        "mm" min/max, "int" quadrature-weighted sum (one per integrate).  Sharing keeps the
+        values each reduction's record receives unchanged while a reduction set such as
+        sum + mean + variance + norm_l2 of one value costs two accumulators instead of six."""
+        accs: List[Tuple[str, str, Optional[int]]] = []
+        index: Dict[Tuple[str, str], int] = {}
+        uses: Dict[int, List[int]] = {}
+
+        def get(kind: str, src: str, ri: Optional[int] = None) -> int:
+            key = (kind, src if kind != "int" else f"{src}#{ri}")
+            if key not in index:
+                index[key] = len(accs)
+                accs.append((kind, src, ri))
+            return index[key]
+
+        for ri, nid in enumerate(self.group.reductions):
+            node = self.graph.node(nid)
+            src, op = node.input_ids[0], node.op_name
+            if op == "variance":
+                uses[ri] = [get("s1", src), get("s2", src)]
+            elif op == "norm_l2":
+                uses[ri] = [get("s2", src)]
+            elif op == "min_max":
+                uses[ri] = [get("mm", src)]
+            elif op == "integrate":
+                uses[ri] = [get("int", src, ri)]
+            else:
+                uses[ri] = [get("s1", src)]
+        return accs, uses
+
     def _write_records(self, A, ndet: int, ind: str) -> None:
         flag = getattr(self, "_flag", False)
         if flag:
@@ -1222,10 +1267,16 @@ class KernelGen:
             cond = " || ".join(f"A.dflag{i} > 0.0f" for i in range(ndet))
             A(f"{ind}if ({cond}) __hip_atomic_store((double*)p.detflag, 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);")
             return
+        _, uses = self.accumulators()
         for ri, nid in enumerate(self.group.reductions):
             op = self.graph.node(nid).op_name
-            fn = {"variance": "block_var", "min_max": "block_mm"}.get(op, "block_sum")
-            A(f"{ind}{fn}(A.r{ri}, red_scratch, p.red{ri} + (i64)rb * 4);")
+            rec = f"p.red{ri} + (i64)rb * 4"
+            if op == "variance":
+                k1, k2 = uses[ri]
+                A(f"{ind}{{ RVar v_; v_.s1 = A.a{k1}; v_.s2 = A.a{k2}; block_var(v_, red_scratch, {rec}); }}")
+            else:
+                fn = "block_mm" if op == "min_max" else "block_sum"
+                A(f"{ind}{fn}(A.a{uses[ri][0]}, red_scratch, {rec});")
 
     def _walk(self, e: "_Emit") -> None:
         pos = e.origin()
@@ -1268,21 +1319,21 @@ class KernelGen:
                 st = "st_bf16" if bf else "st_f32"
                 cast = "(unsigned short*)" if bf else "(float*)"
                 body.append(f"{st}({cast}p.out{oi}, vox * {C} + {c}, {v});")
-        for ri, nid in enumerate([] if flag else self.group.reductions):
-            node = self.graph.node(nid)
-            vals = e.val(node.input_ids[0], pos)
-            if node.op_name == "integrate":
+        for k, (kind, src, ri) in enumerate([] if flag else self.accumulators()[0]):
+            vals = e.val(src, pos)
+            if kind == "int":
+                node = self.graph.node(self.group.reductions[ri])
                 dxs = _dx3(node.params.get("dx", 1.0))
                 meth = 0 if node.params.get("method", "simpson") == "simpson" else 1
                 w = (f"quad_weight(z, N0, {_flt(dxs[0])}, {meth}) * quad_weight(y, N1, {_flt(dxs[1])}, {meth})"
                      f" * quad_weight(x, N2, {_flt(dxs[2])}, {meth})")
-                body.append(f"A.r{ri}.add({vals[0]} * ({w}));")
-            elif node.op_name == "norm_l2":
+                body.append(f"A.a{k}.add({vals[0]} * ({w}));")
+            elif kind == "s2":
                 for v in vals:
-                    body.append(f"rsum_add_sq(A.r{ri}, {v});")
+                    body.append(f"rsum_add_sq(A.a{k}, {v});")
             else:
                 for v in vals:
-                    body.append(f"A.r{ri}.add({v});")
+                    body.append(f"A.a{k}.add({v});")
         for ki, nid in enumerate(self._det_inputs() if flag else []):
             vals = e.val(nid, pos)
             body.append(f"if (fabsf({vals[0]}) < 1e-12f || fabsf({vals[4]}) < 1e-12f || "
@@ -1393,10 +1444,8 @@ class KernelGen:
             pf.append("    int n0, n1, n2, ylo, yhi, xlo, xhi, nbt;\n    int bstart[8];\n    int boxes[36];")
         lines.append("struct Params {\n" + "\n".join(pf) + "\n};\n")
         acc = []
-        for ri, nid in enumerate(g.reductions):
-            op = self.graph.node(nid).op_name
-            ty = {"variance": "RVar", "min_max": "RMinMax"}.get(op, "RSum")
-            acc.append(f"    {ty} r{ri};")
+        for k, (kind, _, _) in enumerate(self.accumulators()[0]):
+            acc.append(f"    {'RMinMax' if kind == 'mm' else 'RSum'} a{k};")
         acc += [f"    float dflag{i};" for i in range(ndet)]
         acc += [f"    float sc{i};" for i in range(nd)]
         acc.append("    float detreg;")
@@ -1457,8 +1506,8 @@ class KernelGen:
             # the flag is an "any": once one block has set it, blocks that start later stop
             A("    if (__hip_atomic_load((double*)p.detflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0.0) return;")
         A("    Acc A;")
-        for ri in range(len(g.reductions)):
-            A(f"    A.r{ri}.init();")
+        for k in range(len(self.accumulators()[0])):
+            A(f"    A.a{k}.init();")
         for i in range(ndet):
             A(f"    A.dflag{i} = 0.0f;")
         # the global +1e-12 I regularisation of determinants, decided by the flag pass
@@ -1491,8 +1540,8 @@ class KernelGen:
             # XCD-aware block remap: XCD k (hardware block id % 8) walks a contiguous tile range
             A("    const int b = blockIdx.x, nb = p.nblocks, per = nb >> 3, rem = nb & 7;")
         A("    Acc A;")
-        for ri in range(len(g.reductions)):
-            A(f"    A.r{ri}.init();")
+        for k in range(len(self.accumulators()[0])):
+            A(f"    A.a{k}.init();")
         for i in range(ndet):
             A(f"    A.dflag{i} = 0.0f;")
         # the global +1e-12 I regularisation of determinants, decided by the flag pass
@@ -1525,6 +1574,7 @@ class KernelGen:
             A("        ly0 = fl % RP;")
             A("        lx = !fact ? 0 : (lo ? fd : TX - 1 - fd);")
             A("    }")
+            A("    const bool xface_tile = (XLO_IN && x0 == 0) || (XHI_IN && x0 + TX == N2);")
         A("    const bool ex = (x0 < REDGE) || (x0 + TX - 1 + REDGE > N2 - 1);")
         A("    const bool ey = (y0 < REDGE) || (y0 + TY - 1 + REDGE > N1 - 1);")
         # partial tiles (last tile of an axis not divisible by the tile) use the gather
@@ -1546,18 +1596,42 @@ class KernelGen:
         lds = self.lds_sources()
         if self.use_lds:
             self._emit_loaders(A)
-            self._emit_prologue(A)
-        A("    for (int z = zb; z < ze; z += 2) {")
-        self._emit_step(A, names, "A", 0)
-        A("        if (z + 1 >= ze) break;")
-        A("        {")
-        A("        const int z_ = z + 1;")
-        self._emit_step(A, names, "B", 1, zname="z_")
-        A("        }")
-        A("    }")
+        if self.use_lds and self.fw:
+            # Wave-uniform split (NT is a multiple of 64): the compute waves run straight-line,
+            # unconditional loads, so the compiler waits on load counts, never on zero; the face
+            # wave loads only its own face cells in x-face tiles and elsewhere just meets the
+            # barriers: one for the prologue and one per plane, as the compute waves do.
+            A("    if (__builtin_amdgcn_readfirstlane((int)fwave)) {")
+            A("      if (!xface_tile) {")
+            A("        for (int zz_ = zb - 1; zz_ < ze; ++zz_) __syncthreads();")
+            A("      } else {")
+            self._role = "face"
+            self._emit_march(A, names)
+            A("      }")
+            A("    } else {")
+            self._role = "compute"
+            self._emit_march(A, names)
+            A("    }")
+            self._role = "all"
+        else:
+            self._emit_march(A, names)
         self._write_records(A, ndet, "    ")
         A("}")
         return "\n".join(out)
+
+    def _emit_march(self, A, names) -> None:
+        """Prologue and z-loop for the current role (unrolled by four: the register ring)."""
+        if self.use_lds:
+            self._emit_prologue(A)
+        A("    for (int z = zb; z < ze; z += 4) {")
+        self._emit_step(A, names, "A", 0, t=0)
+        for t in (1, 2, 3):
+            A(f"        if (z + {t} >= ze) break;")
+            A("        {")
+            A(f"        const int z{t}_ = z + {t};")
+            self._emit_step(A, names, "AB"[t % 2], t % 2, zname=f"z{t}_", t=t)
+            A("        }")
+        A("    }")
 
     def _halo_count(self, i: int) -> int:
         L_ = self.src_layout[i]
@@ -1577,13 +1651,14 @@ class KernelGen:
             L_["KH"] = KH
             # own cells: plane-relative offsets; cells past the grid edge (partial tiles) load
             # their periodic images, like the halo (band mode), or the edge value (variants)
-            A(f"    int og{i}[VPT];")
+            A(f"    unsigned og{i}[VPT];")
             A("    #pragma unroll")
             A("    for (int k = 0; k < VPT; ++k)")
             fn = "wrapm" if self.edge_mode == "band" else "clampc"
             A(f"        og{i}[k] = ({fn}(y0 + ly0 + k * RP, N1) * N2 + {fn}(x0 + lx, N2)) * {C};")
             if KH:
-                A(f"    int hg{i}[{KH}], hl{i}[{KH}];")
+                A(f"    unsigned hg{i}[{KH}];")
+                A(f"    int hl{i}[{KH}];")
                 A("    #pragma unroll")
                 A(f"    for (int j = 0; j < {KH}; ++j) {{")
                 A(f"        const int e = tid < NT ? tid + j * NT : {HT};")
@@ -1601,35 +1676,55 @@ class KernelGen:
                 A(f"        hl{i}[j] = e < {HT} ? (ch * {RH + 1} + r) * {W} + col : {((C - 1) * (RH + 1) + RH) * W};")
                 A("    }")
             for buf in ("A", "B"):
-                A(f"    float {buf}o{i}[VPT][{C}];")
+                if L_["mode"] != "queue":
+                    A(f"    float {buf}o{i}[VPT][{C}];")
                 if KH:
                     A(f"    float {buf}h{i}[{KH}];")
             if L_["mode"] == "queue":
-                A(f"    float qm{i}[VPT][{C}], q0{i}[VPT][{C}];")
+                # planes z-1, z, z+1 and the one in flight live in a ring of four register sets
+                # whose roles advance one slot per plane; the z-loop is unrolled by four, so
+                # every role is a fixed register and nothing is copied between planes
+                A(f"    float " + ", ".join(f"R{i}_{r}[VPT][{C}]" for r in range(4)) + ";")
 
-    def _fetch(self, A, i: int, buf: str, zexpr: str, indent: str = "    ") -> None:
+    def _fetch(self, A, i: int, buf: str, zexpr: str, indent: str = "    ",
+               own: Optional[str] = None) -> None:
+        """Load plane ``zexpr`` of source i: own cells into ``own`` (default ``{buf}o{i}``),
+        halo cells into ``{buf}h{i}``."""
+        own = own or f"{buf}o{i}"
         L_ = self.src_layout[i]
         C, KH = L_["C"], L_.get("KH", 0)
-        A(f"{indent}{{ const IDX pb = (IDX)wrapm({zexpr}, N0) * (N1 * N2 * {C});")
+        face = getattr(self, "_role", "all") == "face"
+        if face and L_["mode"] != "queue":
+            return          # the face wave reads LDS-ring sources from LDS only
+        if face:
+            KH = 0          # ... and needs the own cells of queue sources, never their halo
+        # a uniform plane pointer plus loop-invariant unsigned lane offsets: the loads take the
+        # scalar-base + 32-bit offset form and need no per-load address arithmetic
+        A(f"{indent}{{ const float* __restrict__ pl_ = p.src{i} + (IDX)wrapm({zexpr}, N0) * (N1 * N2 * {C});")
         A(f"{indent}  #pragma unroll")
         A(f"{indent}  for (int k = 0; k < VPT; ++k)")
         A(f"{indent}    #pragma unroll")
-        A(f"{indent}    for (int c = 0; c < {C}; ++c) {buf}o{i}[k][c] = p.src{i}[pb + og{i}[k] + c];")
+        A(f"{indent}    for (int c = 0; c < {C}; ++c) {own}[k][c] = pl_[og{i}[k] + (unsigned)c];")
         if KH:
             A(f"{indent}  #pragma unroll")
-            A(f"{indent}  for (int j = 0; j < {KH}; ++j) {buf}h{i}[j] = p.src{i}[pb + hg{i}[j]];")
+            A(f"{indent}  for (int j = 0; j < {KH}; ++j) {buf}h{i}[j] = pl_[hg{i}[j]];")
         A(f"{indent}}}")
 
-    def _xf_own(self, A, i: int, buf: str, indent: str = "    ") -> None:
+    def _xf_own(self, A, i: int, buf: str, indent: str = "    ", own: Optional[str] = None) -> None:
         """Apply source i's load transform to its own-cell buffer in place."""
+        own = own or f"{buf}o{i}"
         if i in self.xform:
             A(f"{indent}#pragma unroll")
-            A(f"{indent}for (int k = 0; k < VPT; ++k) {buf}o{i}[k][0] = xf{i}({buf}o{i}[k][0]);")
+            A(f"{indent}for (int k = 0; k < VPT; ++k) {own}[k][0] = xf{i}({own}[k][0]);")
 
-    def _store(self, A, i: int, buf: str, slot: str, indent: str = "    ", own_done: bool = False) -> None:
+    def _store(self, A, i: int, buf: str, slot: str, indent: str = "    ", own_done: bool = False,
+               own: Optional[str] = None) -> None:
+        own = own or f"{buf}o{i}"
         L_ = self.src_layout[i]
+        if getattr(self, "_role", "all") == "face":
+            return          # the compute waves fill LDS, the face wave only reads it
         if not own_done:
-            self._xf_own(A, i, buf, indent)
+            self._xf_own(A, i, buf, indent, own)
         if not L_["lds"]:
             return
         C, H, KH = L_["C"], L_["H"], L_.get("KH", 0)
@@ -1638,7 +1733,7 @@ class KernelGen:
         A(f"{indent}  #pragma unroll")
         A(f"{indent}  for (int k = 0; k < VPT; ++k)")
         A(f"{indent}    #pragma unroll")
-        A(f"{indent}    for (int c = 0; c < {C}; ++c) ls[(c * {RH + 1} + ly0 + k * RP + {H}) * {W} + lx + {H}] = {buf}o{i}[k][c];")
+        A(f"{indent}    for (int c = 0; c < {C}; ++c) ls[(c * {RH + 1} + ly0 + k * RP + {H}) * {W} + lx + {H}] = {own}[k][c];")
         if KH:
             xf = (lambda v: f"xf{i}({v})") if i in self.xform else (lambda v: v)
             A(f"{indent}  #pragma unroll")
@@ -1652,32 +1747,74 @@ class KernelGen:
         for i, L_ in enumerate(self.src_layout):
             C = L_["C"]
             if L_["mode"] == "queue":
-                self._fetch(A, i, "A", "zb - 1")
-                self._xf_own(A, i, "A")
-                A(f"    for (int k = 0; k < VPT; ++k) for (int c = 0; c < {C}; ++c) qm{i}[k][c] = Ao{i}[k][c];")
-                self._fetch(A, i, "A", "zb")
-                self._xf_own(A, i, "A")
-                A(f"    for (int k = 0; k < VPT; ++k) for (int c = 0; c < {C}; ++c) q0{i}[k][c] = Ao{i}[k][c];")
-                self._store(A, i, "A", "0", own_done=True)
-            else:
-                A("    for (int k = -HZ; k <= HZ; ++k) {")
-                self._fetch(A, i, "A", "zb + k", "        ")
-                self._store(A, i, "A", f"(unsigned)(k + HZ) % {L_['D']}u", "        ")
-                A("    }")
+                # ring slot r holds plane zb - 1 + r
+                self._fetch(A, i, "A", "zb - 1", own=f"R{i}_0")
+                self._xf_own(A, i, "A", own=f"R{i}_0")
+                self._fetch(A, i, "A", "zb", own=f"R{i}_1")
+                self._xf_own(A, i, "A", own=f"R{i}_1")
+                self._store(A, i, "A", "0", own_done=True, own=f"R{i}_1")
+                self._fetch(A, i, "A", "zb + 1", own=f"R{i}_2")
+                self._fetch(A, i, "B", "zb + 2", own=f"R{i}_3")
+                continue
+            if getattr(self, "_role", "all") == "face":
+                continue
+            A("    for (int k = -HZ; k <= HZ; ++k) {")
+            self._fetch(A, i, "A", "zb + k", "        ")
+            self._store(A, i, "A", f"(unsigned)(k + HZ) % {L_['D']}u", "        ")
+            A("    }")
             self._fetch(A, i, "A", f"zb + {self._lead(i)}")
             self._fetch(A, i, "B", f"zb + 1 + {self._lead(i)}")
         A("    __syncthreads();")
 
-    def _emit_step(self, A, names, buf: str, sq: int, zname: str = "z") -> None:
+    def _emit_step(self, A, names, buf: str, sq: int, zname: str = "z", t: int = 0) -> None:
+        """One plane of the z-march at ring phase ``t`` (0..3): queue source i reads plane z-1
+        from ring slot t, z from t+1 and z+1 from t+2 (mod 4), and the plane fetched after the
+        step, z+3, replaces slot t."""
         ind = "        "
         z = zname
+        ring = lambda i, d: f"R{i}_{(t + d) % 4}"
         A(f"{ind}{{ const int s = {z} - zb + HZ;")
         A(f"{ind}  const bool ez = ({z} < REDGE) || ({z} + REDGE > N0 - 1);")
         # queue sources: the next plane enters the register queue now, transformed
         for i in self.queue_sources():
-            self._xf_own(A, i, buf, ind + "  ")
-        if self.fw:
-            A(f"{ind}  if (!fwave) {{")
+            self._xf_own(A, i, buf, ind + "  ", own=ring(i, 2))
+        role = getattr(self, "_role", "all")
+        qa = "".join(f", {ring(i, 0)}[k], {ring(i, 1)}[k], {ring(i, 2)}[k]" for i in self.queue_sources())
+        call = f"(p, A, s, {sq}, {z}, y, x, ly, lx, y0, x0, vox{qa});"
+        if role != "face":
+            self._emit_interior(A, names, ind, z, call)
+        if self.fw and role == "face":
+            A(f"{ind}  if (fact) {{")
+            A(f"{ind}  #pragma unroll")
+            A(f"{ind}  for (int k = 0; k < VPT; ++k) {{")
+            A(f"{ind}    const int ly = ly0 + k * RP, y = y0 + ly, x = x0 + lx;")
+            A(f"{ind}    if (!ez && y >= YLO && y < YHI) {{")
+            A(f"{ind}      const IDX vox = (IDX){z} * (N1 * N2) + (IDX)(unsigned)(y * N2 + x);")
+            first = True
+            for side in self.xface_sides():
+                for d in range(self.redge):
+                    xc = d if side == "lo" else self.grid[2] - 1 - d
+                    A(f"{ind}      {'if' if first else 'else if'} (x == {xc}) {self._variant_name(('xface', side, d))}{call}")
+                    first = False
+            A(f"{ind}    }}")
+            A(f"{ind}  }}")
+            A(f"{ind}  }}")
+        if self.use_lds:
+            for i, L_ in enumerate(self.src_layout):
+                q = L_["mode"] == "queue"
+                slot = str(sq ^ 1) if q else f"(unsigned)(s + HZ + 1) % {L_['D']}u"
+                self._store(A, i, buf, slot, ind + "  ", own_done=q, own=ring(i, 2) if q else None)
+            A(f"{ind}  __syncthreads();")
+            # always fetch (clamped to the last plane this block needs, an L2 hit) so that
+            # every path issues the same loads and the compiler can wait on counts, not zero
+            for i, L_ in enumerate(self.src_layout):
+                ld = self._lead(i)
+                self._fetch(A, i, buf, f"min({z} + 2 + {ld}, ze - 1 + {ld})", ind + "  ",
+                            own=ring(i, 0) if L_["mode"] == "queue" else None)
+        A(f"{ind}}}")
+
+    def _emit_interior(self, A, names, ind: str, z: str, call: str) -> None:
+        """The compute waves' evaluation of one plane (interior variant, optional staging)."""
         A(f"{ind}  #pragma unroll")
         A(f"{ind}  for (int k = 0; k < VPT; ++k) {{")
         A(f"{ind}    const int ly = ly0 + k * RP, y = y0 + ly, x = x0 + lx;")
@@ -1685,9 +1822,7 @@ class KernelGen:
         if staged:
             A(f"{ind}    A.vflag = 0;")
         A(f"{ind}    if (y < N1 && x < N2) {{")
-        A(f"{ind}      const IDX vox = ((IDX){z} * N1 + y) * N2 + x;")
-        qa = "".join(f", qm{i}[k], q0{i}[k], {buf}o{i}[k]" for i in self.queue_sources())
-        call = f"(p, A, s, {sq}, {z}, y, x, ly, lx, y0, x0, vox{qa});"
+        A(f"{ind}      const IDX vox = (IDX){z} * (N1 * N2) + (IDX)(unsigned)(y * N2 + x);")
         if self.edge_mode == "band" and ("lds", False, False) in names:
             # z-edge planes, the y face band and corners belong to the tail blocks; the
             # x-face band to the face wave
@@ -1731,39 +1866,6 @@ class KernelGen:
             A(f"{ind}      }}")
             A(f"{ind}    }}")
         A(f"{ind}  }}")
-        if self.fw:
-            A(f"{ind}  }} else if (fact) {{")
-            A(f"{ind}  #pragma unroll")
-            A(f"{ind}  for (int k = 0; k < VPT; ++k) {{")
-            A(f"{ind}    const int ly = ly0 + k * RP, y = y0 + ly, x = x0 + lx;")
-            A(f"{ind}    if (!ez && y >= YLO && y < YHI) {{")
-            A(f"{ind}      const IDX vox = ((IDX){z} * N1 + y) * N2 + x;")
-            first = True
-            for side in self.xface_sides():
-                for d in range(self.redge):
-                    xc = d if side == "lo" else self.grid[2] - 1 - d
-                    A(f"{ind}      {'if' if first else 'else if'} (x == {xc}) {self._variant_name(('xface', side, d))}{call}")
-                    first = False
-            A(f"{ind}    }}")
-            A(f"{ind}  }}")
-            A(f"{ind}  }}")
-        if self.use_lds:
-            for i, L_ in enumerate(self.src_layout):
-                slot = str(sq ^ 1) if L_["mode"] == "queue" else f"(unsigned)(s + HZ + 1) % {L_['D']}u"
-                self._store(A, i, buf, slot, ind + "  ", own_done=L_["mode"] == "queue")
-            A(f"{ind}  __syncthreads();")
-            for i in self.queue_sources():
-                C = self.src_layout[i]["C"]
-                A(f"{ind}  #pragma unroll")
-                A(f"{ind}  for (int k = 0; k < VPT; ++k)")
-                A(f"{ind}    #pragma unroll")
-                A(f"{ind}    for (int c = 0; c < {C}; ++c) {{ qm{i}[k][c] = q0{i}[k][c]; q0{i}[k][c] = {buf}o{i}[k][c]; }}")
-            # always fetch (clamped to the last plane this block needs, an L2 hit) so that
-            # every path issues the same loads and the compiler can wait on counts, not zero
-            for i in range(len(self.src_layout)):
-                ld = self._lead(i)
-                self._fetch(A, i, buf, f"min({z} + 2 + {ld}, ze - 1 + {ld})", ind + "  ")
-        A(f"{ind}}}")
 
 
 # ------------------------------------------------------------------------------ compiled plans
@@ -1923,8 +2025,7 @@ class FusedExecutor:
 
     def prepare(self, graph: Graph, values: Dict[str, Any], wanted: Sequence[str],
                 out_dtype: Optional[torch.dtype] = None, schedule: Optional[Schedule] = None):
-        shapes = tuple(sorted((k, tuple(v.shape) if isinstance(v, torch.Tensor) else None)
-                              for k, v in values.items()))
+        shapes = tuple(sorted((k, _value_signature(v)) for k, v in values.items()))
         key = graph.canonical_key((shapes, tuple(sorted(wanted)), str(out_dtype),
                                    schedule.as_tuple() if schedule else None,
                                    id(self.schedule_fn), self.edge_mode or EDGE_MODE))

@@ -28,9 +28,12 @@ _DB_LOCK = threading.Lock()
 # threads must cover whole tile rows: NT % TX == 0 and TY a multiple of NT / TX
 CANDIDATES: List[Schedule] = [
     Schedule(tx, ty, zc, nt)
-    for tx in (32, 64, 128) for ty in (4, 8, 16) for zc in (32, 64, 128, 256) for nt in (128, 256, 512)
+    for tx in (32, 64, 128) for ty in (4, 8, 16) for zc in (32, 64, 128, 256, 512) for nt in (128, 256, 512)
     if nt % tx == 0 and ty % (nt // tx) == 0 and 1 <= tx * ty // nt <= 4
 ]
+# the schedules the learned model was trained and is evaluated on (its timing rows predate the
+# 512-plane z-chunks, which only the exhaustive search times)
+MODEL_CANDIDATES: List[Schedule] = [c for c in CANDIDATES if c.zc <= 256]
 # a smaller grid of candidates for the training-data sweep (the heuristic schedule of each
 # grid is always added)
 DATA_CANDIDATES: List[Schedule] = [c for c in CANDIDATES if c.zc in (64, 256) or (c.zc == 128 and c.tx == 64)]
@@ -61,20 +64,40 @@ def save_db(db: Dict[str, list]) -> None:
 
 
 def group_signature(group, graph) -> str:
-    """Structure of a fused group (ops, edges, params, grid) -- the tuning DB key."""
+    """Structure of a fused group (ops, edges, params, grid) -- the tuning DB key.
+
+    Nodes are listed in a canonical order that depends only on the structure, not on the order
+    in which a topological sort visits them: by depth inside the group, then by op name,
+    params and the labels of their inputs (assigned in that same order)."""
     idx = {}
     parts = [f"grid={tuple(group.grid)}"]
     for i, s in enumerate(group.sources):
         idx[s] = f"s{i}"
     for i, d in enumerate(group.dscalars):
         idx[d] = f"d{i}"
-    for i, nid in enumerate(group.nodes):
-        node = graph.node(nid)
-        idx[nid] = f"n{i}"
-        ins = ",".join(idx.get(x, "c") for x in node.input_ids)
-        ps = ",".join(f"{k}={v!r}" for k, v in sorted(node.params.items())
-                      if isinstance(v, (int, float, str, bool, tuple, list)))
-        parts.append(f"{node.op_name}({ins})[{ps}]")
+    members = set(group.nodes)
+    depth: Dict[str, int] = {}
+    for nid in group.nodes:                     # group.nodes is topologically ordered
+        depth[nid] = 1 + max([depth[x] for x in graph.node(nid).input_ids if x in members],
+                             default=0)
+
+    def params_of(node) -> str:
+        return ",".join(f"{k}={v!r}" for k, v in sorted(node.params.items())
+                        if isinstance(v, (int, float, str, bool, tuple, list)))
+
+    n = 0
+    for level in sorted(set(depth.values())):
+        rows = []
+        for nid in group.nodes:
+            if depth[nid] != level:
+                continue
+            node = graph.node(nid)
+            ins = ",".join(idx.get(x, "c") for x in node.input_ids)
+            rows.append((node.op_name, params_of(node), ins, nid))
+        for op, ps, ins, nid in sorted(rows):
+            idx[nid] = f"n{n}"
+            n += 1
+            parts.append(f"{op}({ins})[{ps}]")
     parts.append("out=" + ",".join(idx[o] for o in group.outputs))
     return "|".join(parts)
 
@@ -303,7 +326,7 @@ class ScheduleModel:
 
     def __init__(self, mlp=None, cands: Sequence[Schedule] = None):
         self.mlp = mlp
-        self.cands = list(cands or CANDIDATES)
+        self.cands = list(cands or MODEL_CANDIDATES)
 
     def fit(self, rows: List[Dict[str, Any]], device="cuda", epochs: int = 40, seed: int = 0,
             weight_decay: float = 1e-3):
@@ -427,7 +450,7 @@ def train_and_report(rows: List[Dict[str, Any]], device="cuda", epochs: int = 40
     """Fit on 4/5 of the graphs, report regret / gain on the held-out fifth (pick = the top
     prediction, and the best of the top three as the shortlist search would time them)."""
     train, test = split_by_graph(rows)
-    model = ScheduleModel(cands=CANDIDATES)
+    model = ScheduleModel(cands=MODEL_CANDIDATES)
     losses = model.fit(train, device=device, epochs=epochs, seed=seed)
     rep = {"rows_train": len(train), "rows_test": len(test), "epochs": epochs,
            "graphs": len({r.get("graph") for r in rows}),

@@ -151,9 +151,10 @@ def test_tail_blocks_join_the_main_grid_unless_registers_grow(expr):
         assert F.kernel_registers(F.compile_source(comp[0].source)[0])["fused_tail"] > 0
 
 
-def test_register_rule_splits_the_tail_before_dropping_staging():
-    # the five-sink tensor group (band edge mode): staged and merged it needs > 128 VGPRs; the
-    # plan first moves the edge band to its own kernel, then stores directly, and ends at <= 128
+def test_register_rule_splits_the_tail_before_dropping_staging(monkeypatch):
+    # the five-sink tensor group (band edge mode) is staged with its tail in the main grid; the
+    # rule keeps that when the kernel fits the register limit, and otherwise first moves the edge
+    # band to its own kernel, then stores directly.  The limit is lowered to exercise each step.
     f = torch.zeros((96, 96, 160))
     exprs = ["eigenvalues(hessian(f))", "determinant(hessian(f))", "trace(hessian(f))",
              "deviatoric(hessian(f))", "mean_curvature(f)"]
@@ -161,15 +162,29 @@ def test_register_rule_splits_the_tail_before_dropping_staging():
     g, sinks, *_ = compile_expressions(exprs, boundary="periodic", field_map={"f": f})
     values = {n: g.node(n).params["value"] for n in g._order if g.node(n).op_name == "_constant"}
     steps, info = F.plan_graph(g, values, sinks)
-    gen = F.KernelGen(g, steps[0].group, info, F.Schedule(32, 8, 16, 128), {}, {})
-    assert gen.stage, "the 9-channel deviatoric is eligible for staging"
     regs = lambda s: F.kernel_registers(F.compile_source(s)[0])["fused_kernel"]
-    before = regs(gen.source())
-    src = F.FusedExecutor._fit_registers(gen, gen.source())
-    after = regs(src)
-    assert before > F.MERGED_TAIL_MAX_VGPRS >= after
+
+    def fitted(limit):
+        monkeypatch.setattr(F, "MERGED_TAIL_MAX_VGPRS", limit)
+        gen = F.KernelGen(g, steps[0].group, info, F.Schedule(32, 8, 16, 128), {}, {})
+        assert gen.stage, "the 9-channel deviatoric is eligible for staging"
+        return gen, gen.source(), F.FusedExecutor._fit_registers(gen, gen.source())
+
+    gen, before_src, src = fitted(128)
+    before = regs(before_src)
+    if before <= 128:
+        assert src == before_src and gen.stage and not gen.split_tail
+    # a limit just below the merged, staged kernel: the tail moves out first, staging stays
+    gen, _, src = fitted(before - 1)
+    split_regs = regs(src)
     assert gen.split_tail and "fused_tail" in F.kernel_registers(F.compile_source(src)[0])
+    if split_regs <= before - 1:
+        assert gen.stage
+    # a limit below the split, staged kernel: staging goes too
+    gen, _, src = fitted(min(split_regs, before) - 1)
+    assert gen.split_tail and not gen.stage
     # a light staged group keeps both its staging and its in-grid tail
+    monkeypatch.setattr(F, "MERGED_TAIL_MAX_VGPRS", 128)
     g2, s2, *_ = compile_expressions(["hessian(f)"], boundary="periodic", field_map={"f": f})
     values = {n: g2.node(n).params["value"] for n in g2._order if g2.node(n).op_name == "_constant"}
     steps2, info2 = F.plan_graph(g2, values, s2)

@@ -169,6 +169,36 @@ def test_channel_helpers():
     assert_2x(ops.select_component(u, 1), o64, o32, "select_component")
 
 
+def test_remaining_channel_helpers():
+    """stack_components, div_last (both divisor ranks), neg_stack3, scale_eye and
+    velocity_dot_grad (stenpy_engine.py:2247-2313), directly and inside a graph."""
+    a, b, c = field(SHAPE, 31), field(SHAPE, 32), field(SHAPE, 33)
+    u = vfield(SHAPE, seeds=(34, 35, 36))
+    o64, o32 = ref(O.stack_components, a, b, c)
+    assert_2x(ops.stack_components(a, b, c), o64, o32, "stack_components")
+    d = (b.abs() + 0.25).contiguous()
+    o64, o32 = ref(O.div_last, u, d)
+    assert_2x(ops.div_last(u, d), o64, o32, "div_last rank-1 divisor")
+    w = (u.abs() + 0.5).contiguous()
+    o64, o32 = ref(O.div_last, u, w)
+    assert_2x(ops.div_last(u, w), o64, o32, "div_last same-rank divisor")
+    o64, o32 = ref(O.neg_stack3, a * 40, b * 40)
+    assert_2x(ops.neg_stack3(a * 40, b * 40), o64, o32, "neg_stack3")
+    o64, o32 = ref(O.scale_eye, a, 3)
+    got = ops.scale_eye(a, 3)
+    assert got.shape == SHAPE + (3, 3)
+    assert_2x(got, o64, o32, "scale_eye")
+    o64, o32 = ref(O.velocity_dot_grad, a, c)
+    assert_2x(ops.velocity_dot_grad(a, c), o64, o32, "velocity_dot_grad")
+    # the same helpers reached from an expression through the eager executor
+    from oracle import graph_eval
+    g, sink, *_ = sp.compile_expression("norm_last(stack_components(f, g, h))",
+                                        field_map={"f": a, "g": b, "h": c})
+    res = sp.Runtime(device="cuda", fusion_enabled=False).run(g)[sink]
+    assert_2x(res, graph_eval.evaluate(g, torch.float64)[sink],
+              graph_eval.evaluate(g, torch.float32)[sink], "helpers in a graph")
+
+
 @pytest.mark.parametrize("kind", ["sum", "mean", "norm_l2", "variance", "min_max"])
 def test_reductions(kind):
     f = field((32, 33, 34), 25) + 0.5

@@ -1,0 +1,147 @@
+"""Out-of-core evaluation against the oracle: ``stream_graph``, ``stream_execute`` and
+``Pipeline.run`` (eager and ``stream_i``), 2x error rule on the stitched output and on the
+streamed reductions.
+
+The oracle of a streamed run is the reference's own chunked semantics
+(stenpy_engine.py:647-695, 697-888): every slab of ``slab`` planes is padded with ``r`` planes
+read with periodic wrap-around (periodic) or truncated at the ends (otherwise), the graph is
+evaluated on the padded slab, and the halo is cropped.  For graphs whose edge handling is the
+boundary mode's (everything but the one-sided ops that ignore it) this equals the whole-field
+evaluation, which is checked too.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import stenpy_amd as sp
+from stenpy_amd.stream import GeneratorField, NpyLazyField, stream_execute, stream_graph
+from oracle import graph_eval
+from tests._util import assert_2x, errs
+
+pytestmark = pytest.mark.gpu
+
+C4_EXPR = "f**2 + tanh(f)*exp(f) + laplacian(f)"
+
+
+def oracle_whole(expr, full, boundary, dtype, dx=1.0):
+    g, sink, *_ = sp.compile_expression(expr, dx=dx, boundary=boundary,
+                                        field_map={"f": torch.from_numpy(np.ascontiguousarray(full))})
+    return graph_eval.evaluate(g, dtype)[sink]
+
+
+def oracle_streamed(expr, full, slab, r, boundary, dtype, dx=1.0):
+    """Slab-wise evaluation with the reference's halo reads and crop."""
+    n0 = full.shape[0]
+    outs = []
+    for s in range(0, n0, slab):
+        e = min(s + slab, n0)
+        if boundary == "periodic":
+            rows, lp = np.arange(s - r, e + r) % n0, r
+        else:
+            lo, hi = max(0, s - r), min(n0, e + r)
+            rows, lp = np.arange(lo, hi), s - lo
+        o = oracle_whole(expr, full[rows], boundary, dtype, dx)
+        outs.append(o[lp:lp + e - s])
+    return torch.cat(outs)
+
+
+def check_stream_stats(st, o64, o32, what):
+    """The streamed reductions (L2, sum, mean, min, max) under the 2x rule."""
+    scalars = {"l2": lambda t: torch.linalg.vector_norm(t), "sum": lambda t: t.sum(),
+               "mean": lambda t: t.mean(), "min": lambda t: t.min(), "max": lambda t: t.max()}
+    for k, fn in scalars.items():
+        b = torch.tensor(float(st[k]), dtype=torch.float64)
+        assert_2x(b, fn(o64).double(), fn(o32).double(), f"{what} {k}")
+
+
+@pytest.mark.parametrize("boundary", ["periodic", "neumann"])
+@pytest.mark.parametrize("slab", [7, 16])
+def test_c4_expression_streamed_against_oracle(boundary, slab, tmp_path):
+    shape = (40, 36, 70)
+    lf = GeneratorField(shape, 7)
+    full = lf[0:shape[0]]
+    probe = torch.zeros((1,) + shape[1:])
+    g, sink, _, fm, _ = sp.compile_expression(C4_EXPR, boundary=boundary, field_map={"f": probe})
+    outp = str(tmp_path / "out.npy")
+    st = stream_graph(g, sink, fm["f"], lf, out_path=outp, boundary=boundary, slab_planes=slab)
+    got = torch.from_numpy(np.load(outp))
+    o64 = oracle_streamed(C4_EXPR, full, slab, 1, boundary, torch.float64)
+    o32 = oracle_streamed(C4_EXPR, full, slab, 1, boundary, torch.float32)
+    assert torch.equal(o64, oracle_whole(C4_EXPR, full, boundary, torch.float64))
+    assert_2x(got, o64, o32, f"stream {boundary} slab {slab}")
+    check_stream_stats(st, o64, o32, f"stream {boundary} slab {slab}")
+    assert st["n_chunks"] == -(-shape[0] // slab) and st["halo"] == 1
+
+
+@pytest.mark.parametrize("boundary", ["periodic", "neumann"])
+def test_radius_two_channel_output_streamed(boundary, tmp_path):
+    """``eigenvalues(hessian(f))``: halo of 2 planes, a 3-channel output, both boundaries.
+    With Neumann edges the slab-wise result equals the whole-field one; with periodic edges
+    the one-sided hessian sees wrapped halo planes at the global faces, as the reference's
+    chunked path does, so the oracle is the chunked evaluation."""
+    expr = "eigenvalues(hessian(f))"
+    shape, slab = (30, 24, 40), 8
+    lf = GeneratorField(shape, 12)
+    full = lf[0:shape[0]]
+    probe = torch.zeros((1,) + shape[1:])
+    g, sink, _, fm, _ = sp.compile_expression(expr, boundary=boundary, field_map={"f": probe})
+    outp = str(tmp_path / "ev.npy")
+    st = stream_graph(g, sink, fm["f"], lf, out_path=outp, boundary=boundary, slab_planes=slab)
+    assert st["halo"] == 2
+    got = torch.from_numpy(np.load(outp))
+    assert got.shape == shape + (3,)
+    o64 = oracle_streamed(expr, full, slab, 2, boundary, torch.float64)
+    o32 = oracle_streamed(expr, full, slab, 2, boundary, torch.float32)
+    if boundary == "neumann":
+        assert torch.equal(o64, oracle_whole(expr, full, boundary, torch.float64))
+    assert_2x(got, o64, o32, f"eigenvalues stream {boundary}")
+    check_stream_stats(st, o64, o32, f"eigenvalues stream {boundary}")
+
+
+def test_stream_execute_and_pipeline_stream_i(tmp_path):
+    shape = (33, 20, 48)
+    full = GeneratorField(shape, 21)[0:shape[0]]
+    src = tmp_path / "f.npy"
+    np.save(src, full)
+    expr = "laplacian(f) * tanh(f) + sqrt(f**2 + 1)"
+    o64 = oracle_whole(expr, full, "neumann", torch.float64)
+    o32 = oracle_whole(expr, full, "neumann", torch.float32)
+    out1 = str(tmp_path / "se.npy")
+    st = stream_execute(expr, NpyLazyField(str(src)), out1, boundary="neumann", slab_planes=5)
+    assert_2x(torch.from_numpy(np.load(out1)), o64, o32, "stream_execute")
+    check_stream_stats(st, o64, o32, "stream_execute")
+    p = sp.Pipeline("custom", expr)
+    p.f = str(src)
+    p.boundary = "neumann"
+    p.stream_i = True
+    p.chunk_mb = 6 * shape[1] * shape[2] * 4 / 2 ** 20        # 6-plane slabs
+    out2 = str(tmp_path / "pipe.npy")
+    p.output = out2                                          # setting output runs the pipeline
+    meta = p.value
+    assert meta["n_chunks"] == math.ceil(shape[0] / 6)
+    assert_2x(torch.from_numpy(np.load(out2)), o64, o32, "Pipeline stream_i")
+
+
+def test_pipeline_eager_run_and_operator_dispatch(tmp_path):
+    shape = (18, 22, 30)
+    full = GeneratorField(shape, 5)[0:shape[0]]
+    src = tmp_path / "f.npy"
+    np.save(src, full)
+    expr = "divergence(gradient(f)) + exp(f) * 2"
+    p = sp.Pipeline("laplacian", expr)
+    p.f = str(src)
+    out = str(tmp_path / "o.npy")
+    p.output = out
+    o64 = oracle_whole(expr, full, "periodic", torch.float64)
+    o32 = oracle_whole(expr, full, "periodic", torch.float32)
+    assert_2x(p.value, o64, o32, "Pipeline eager")
+    assert_2x(torch.from_numpy(np.load(out)), o64, o32, "Pipeline eager file")
+    # stenpy.<op>(expr) builds the same pipeline (the op name is only a label)
+    q = sp.gradient("f**2 + f")
+    q.f = torch.from_numpy(full)
+    q.boundary = "neumann"
+    res = q.run()
+    assert_2x(res, oracle_whole("f**2 + f", full, "neumann", torch.float64),
+              oracle_whole("f**2 + f", full, "neumann", torch.float32), "operator dispatch")
