@@ -294,7 +294,7 @@ class BatchRunner:
         starts.append(acc)
         ints = [tx_, ty_, zcn, e.nb] + [0] + [n0, n1, n2, ylo, yhi, xlo, xhi, e.nbt] + starts + flat
         assert len(ints) == _N_INT_FIELDS - 1
-        return struct.pack(f"<{len(ptrs)}Q4ifi8i8i36i", *ptrs, tx_, ty_, zcn, e.nb, 0.0, 0,
+        return struct.pack(f"<{len(ptrs)}Q4i2i8i8i36i", *ptrs, tx_, ty_, zcn, e.nb, *F.Z_ALL,
                            n0, n1, n2, ylo, yhi, xlo, xhi, e.nbt, *starts, *flat)
 
     def _pack(self) -> None:
@@ -403,8 +403,9 @@ class BatchRunner:
             out[w] = v
         return out
 
-    def capture(self, warmup: int = 1) -> "torch.cuda.CUDAGraph":
-        """A hipGraph replaying every launch of the batch (buffers are fixed at plan time)."""
+    def capture(self, warmup: int = 1) -> "F.CapturedGraph":
+        """A hipGraph replaying every launch of the batch (buffers are fixed at plan time); it
+        keeps this runner, and with it the modules and buffers it launches, alive."""
         self.compile()
         for lvl in range(len(self.launches)):
             if self.launches[lvl]["nunits"]:
@@ -418,7 +419,7 @@ class BatchRunner:
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self.run()
-        return g
+        return F.CapturedGraph(g, [self])
 
     def stats(self) -> Dict[str, Any]:
         """Units, grids and algorithmic bytes (each source channel read once, each output

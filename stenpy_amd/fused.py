@@ -1319,7 +1319,10 @@ class KernelGen:
                 st = "st_bf16" if bf else "st_f32"
                 cast = "(unsigned short*)" if bf else "(float*)"
                 body.append(f"{st}({cast}p.out{oi}, vox * {C} + {c}, {v});")
-        for k, (kind, src, ri) in enumerate([] if flag else self.accumulators()[0]):
+        accs = [] if flag else self.accumulators()[0]
+        if accs:
+            body.append("if (z >= p.zr0 && z < p.zr1) {")
+        for k, (kind, src, ri) in enumerate(accs):
             vals = e.val(src, pos)
             if kind == "int":
                 node = self.graph.node(self.group.reductions[ri])
@@ -1334,6 +1337,8 @@ class KernelGen:
             else:
                 for v in vals:
                     body.append(f"A.a{k}.add({v});")
+        if accs:
+            body.append("}")
         for ki, nid in enumerate(self._det_inputs() if flag else []):
             vals = e.val(nid, pos)
             body.append(f"if (fabsf({vals[0]}) < 1e-12f || fabsf({vals[4]}) < 1e-12f || "
@@ -1439,7 +1444,8 @@ class KernelGen:
         pf += [f"    const double* scal{i};" for i in range(nd)]
         if ndet:
             pf.append("    const double* detflag;      // 1.0: any |H_ii| < 1e-12 (from the flag pass)")
-        pf.append("    int tiles_x, tiles_y, zchunks, nblocks;\n    float unused_; int pad_;")
+        pf.append("    int tiles_x, tiles_y, zchunks, nblocks;\n"
+                  "    int zr0, zr1;               // reductions take the planes [zr0, zr1)")
         if self.batched:
             pf.append("    int n0, n1, n2, ylo, yhi, xlo, xhi, nbt;\n    int bstart[8];\n    int boxes[36];")
         lines.append("struct Params {\n" + "\n".join(pf) + "\n};\n")
@@ -1968,16 +1974,32 @@ class CompiledGroup:
         return tx, ty, zcn, tx * ty * zcn
 
 
+class CapturedGraph:
+    """A hipGraph of fused launches together with what its replays need alive: the compiled
+    groups whose code-object modules hold the recorded kernels (and, for a batch, the runner
+    that owns the recorded buffers).  Dropping the executor that captured it is safe."""
+
+    def __init__(self, graph: "torch.cuda.CUDAGraph", keep: Sequence[Any]):
+        self.graph = graph
+        self._keep = list(keep)
+
+    def replay(self) -> None:
+        self.graph.replay()
+
+
+Z_ALL = (0, 2 ** 31 - 1)       # reduction window covering every plane
+
+
 def pack_params(cg: CompiledGroup, srcs: Sequence[torch.Tensor], outs: Sequence[torch.Tensor],
                 reds: Sequence[torch.Tensor], dflags: Sequence[torch.Tensor],
-                scal: Sequence[torch.Tensor], det_reg: float,
-                detflag: Optional[torch.Tensor] = None) -> bytes:
+                scal: Sequence[torch.Tensor], detflag: Optional[torch.Tensor] = None,
+                zwin: Tuple[int, int] = Z_ALL) -> bytes:
     tx, ty, zcn, nb = cg.nblocks()
     ptrs = [t.data_ptr() for t in srcs] + [t.data_ptr() for t in outs] + \
            [t.data_ptr() for t in reds] + [t.data_ptr() for t in dflags] + [t.data_ptr() for t in scal]
     if dflags:
         ptrs.append(detflag.data_ptr())
-    return struct.pack(f"<{len(ptrs)}Q4ifi", *ptrs, tx, ty, zcn, nb, float(det_reg), 0)
+    return struct.pack(f"<{len(ptrs)}Q6i", *ptrs, tx, ty, zcn, nb, int(zwin[0]), int(zwin[1]))
 
 
 # ------------------------------------------------------------------------------ executor
@@ -2064,7 +2086,10 @@ class FusedExecutor:
 
     def run(self, graph: Graph, runtime, seed: Optional[Dict[str, Any]] = None,
             wanted: Optional[Sequence[str]] = None, out_dtype: Optional[torch.dtype] = None,
-            schedule: Optional[Schedule] = None) -> Dict[str, Any]:
+            schedule: Optional[Schedule] = None,
+            red_window: Optional[Tuple[int, int]] = None) -> Dict[str, Any]:
+        """Evaluate ``graph``; ``red_window = (z0, z1)`` restricts every reduction to the
+        planes [z0, z1) of its input (a padded slab's interior, for the out-of-core path)."""
         from .registry import OP_REGISTRY
         results: Dict[str, Any] = dict(seed or {})
         values: Dict[str, Any] = {}
@@ -2083,13 +2108,16 @@ class FusedExecutor:
             if st.kind == "eager":
                 node = graph.node(st.node)
                 ins = [results[i] for i in node.input_ids]
+                if red_window is not None and node.op_name in REDUCTION_OPS:
+                    ins[0] = ins[0][red_window[0]:red_window[1]]
                 results[st.node] = OP_REGISTRY[node.op_name](*ins, **node.params)
                 continue
-            self._run_group(compiled[si], graph, results)
+            self._run_group(compiled[si], graph, results, red_window)
         return results
 
     def capture(self, graph: Graph, runtime, wanted: Sequence[str],
-                out_dtype: Optional[torch.dtype] = None, warmup: int = 1):
+                out_dtype: Optional[torch.dtype] = None, warmup: int = 1,
+                schedule: Optional[Schedule] = None):
         """Capture one evaluation of ``graph`` as a hipGraph.
 
         Returns (graph, results): replaying the graph re-evaluates into the same ``results``
@@ -2101,12 +2129,13 @@ class FusedExecutor:
         side.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(side):
             for _ in range(warmup):
-                self.run(graph, runtime, wanted=wanted, out_dtype=out_dtype)
+                self.run(graph, runtime, wanted=wanted, out_dtype=out_dtype, schedule=schedule)
         torch.cuda.current_stream(self.device).wait_stream(side)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            results = self.run(graph, runtime, wanted=wanted, out_dtype=out_dtype)
-        return g, results
+            results = self.run(graph, runtime, wanted=wanted, out_dtype=out_dtype, schedule=schedule)
+        keep = [cg for plan in self._plans.values() for cg in plan[2].values()]
+        return CapturedGraph(g, keep), results
 
     def prepare_launch(self, cg: CompiledGroup, graph: Graph, results: Dict[str, Any]) -> "Launch":
         """Buffers and packed arguments of one launch of ``cg`` (inputs read from ``results``)."""
@@ -2145,11 +2174,14 @@ class FusedExecutor:
         side = self.side_stream() if cg.gen.split_tail else None
         return Launch(cg, srcs, outs, reds, dflags, scal, nb, side, detflag)
 
-    def _run_group(self, cg: CompiledGroup, graph: Graph, results: Dict[str, Any]) -> None:
+    def _run_group(self, cg: CompiledGroup, graph: Graph, results: Dict[str, Any],
+                   red_window: Optional[Tuple[int, int]] = None) -> None:
         g = cg.group
         n0, n1, n2 = g.grid
         dev = self.device
         ln = self.prepare_launch(cg, graph, results)
+        z0, z1 = (0, n0) if red_window is None else (max(0, red_window[0]), min(n0, red_window[1]))
+        ln.zwin = (z0, z1)
         stream = L.stream_ptr(dev)
         nb = ln.nblocks + cg.gen.boundary_blocks()
         if ln.dflags:
@@ -2164,7 +2196,7 @@ class FusedExecutor:
         for r, ws in zip(g.reductions, ln.reds):
             node = graph.node(r)
             inp = node.input_ids[0]
-            count = n0 * n1 * n2 * max(cg.gen.info[inp].ch, 1)
+            count = (z1 - z0) * n1 * n2 * max(cg.gen.info[inp].ch, 1)
             res = torch.empty(2 if node.op_name == "min_max" else 1, dtype=torch.float64, device=dev)
             descs.append(struct.pack("<QQqii", ws.data_ptr(), res.data_ptr(), count,
                                      RED_KIND[node.op_name], nb))
@@ -2187,10 +2219,11 @@ class Launch:
     nblocks: int
     side: Optional["torch.cuda.Stream"] = None
     detflag: Optional[torch.Tensor] = None
+    zwin: Tuple[int, int] = Z_ALL
 
     def args(self) -> bytes:
-        return pack_params(self.cg, self.srcs, self.outs, self.reds, self.dflags, self.scal, 0.0,
-                           self.detflag)
+        return pack_params(self.cg, self.srcs, self.outs, self.reds, self.dflags, self.scal,
+                           self.detflag, self.zwin)
 
     def launch(self, stream: Optional[int] = None, entry: str = "fused_kernel") -> None:
         """Main kernel (or the flag pass, ``entry="fused_flag"``) on ``stream`` (default:
@@ -2211,17 +2244,26 @@ class Launch:
         self.cg.kernel(tail).launch((nbt, 1, 1), (gen.block_threads(), 1, 1), a, 0, self.side.cuda_stream)
         main.wait_stream(self.side)
 
+    def run_once(self, stream: Optional[int] = None) -> None:
+        """One evaluation of the group's kernels: the determinant flag pass when the group has
+        one (its flag cleared by a fill kernel first), then the main launch."""
+        if self.dflags:
+            self.detflag.zero_()
+            self.launch(stream, entry="fused_flag")
+        self.launch(stream)
+
     def time_ms(self, reps: int = 20, warmup: int = 3, rounds: int = 3) -> float:
-        """Min over ``rounds`` of the mean kernel time, HIP events on the launch stream."""
+        """Min over ``rounds`` of the mean time of ``run_once``, HIP events on the launch
+        stream."""
         s = L.stream_ptr()
         for _ in range(warmup):
-            self.launch(s)
+            self.run_once(s)
         best = float("inf")
         for _ in range(rounds):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
             for _ in range(reps):
-                self.launch(s)
+                self.run_once(s)
             e1.record()
             e1.synchronize()
             best = min(best, e0.elapsed_time(e1) / reps)

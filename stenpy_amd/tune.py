@@ -206,6 +206,33 @@ def _compile_all(sources: List[str], workers: int) -> List[bytes]:
     return [c for c, _ in _POOL.map(F.compile_source, sources)]
 
 
+def _fit_all(gens: List[F.KernelGen], workers: int) -> Tuple[List[str], List[bytes]]:
+    """The executor's register rule (``FusedExecutor._fit_registers``) applied to every
+    candidate, each stage's recompiles batched through the compile pool: kernels above
+    ``MERGED_TAIL_MAX_VGPRS`` first move the edge band to its own kernel, then drop staging.
+    What is timed is what would run."""
+    srcs = [g.source() for g in gens]
+    codes = _compile_all(srcs, workers)
+    over = lambda c: F.kernel_registers(c).get("fused_kernel", 0) > F.MERGED_TAIL_MAX_VGPRS
+    for stage in ("split", "direct"):
+        redo = []
+        for j, (gen, code) in enumerate(zip(gens, codes)):
+            if not over(code):
+                continue
+            if stage == "split" and F.SPLIT_TAIL == "auto" and gen.use_lds and not gen.batched \
+                    and gen.boundary_blocks() > 0 and not gen.split_tail:
+                gen.want_split = True
+                redo.append(j)
+            elif stage == "direct" and gen.stage:
+                gen.stage, gen.stage_off, gen.stage_c = [], {}, 0
+                redo.append(j)
+        if redo:
+            new = _compile_all([gens[j].source() for j in redo], workers)
+            for j, c in zip(redo, new):
+                srcs[j], codes[j] = gens[j].source(), c
+    return srcs, codes
+
+
 def candidates_for(graph, group, info, out_dtypes, cands: Sequence[Schedule] = None):
     gens = []
     cands = list(cands or CANDIDATES)
@@ -224,7 +251,8 @@ def candidates_for(graph, group, info, out_dtypes, cands: Sequence[Schedule] = N
 
 def autotune(graph, runtime, wanted: Sequence[str], seed: Optional[Dict[str, Any]] = None,
              cands: Sequence[Schedule] = None, reps: int = 10, workers: int = 12,
-             update_db: bool = True, out_dtype: Optional[torch.dtype] = None) -> List[Dict[str, Any]]:
+             update_db: bool = True, out_dtype: Optional[torch.dtype] = None,
+             log=None) -> List[Dict[str, Any]]:
     """Schedule search for every fused group of ``graph``; returns timing rows.
 
     ``cands``: candidate list (default: every valid candidate, i.e. exhaustive search), or a
@@ -242,19 +270,13 @@ def autotune(graph, runtime, wanted: Sequence[str], seed: Optional[Dict[str, Any
         base = comp[si]
         group_cands = cands(st.group, graph) if callable(cands) else cands
         gens = candidates_for(graph, st.group, info, base.gen.out_dtypes, group_cands)
-        srcs = [g.source() for g in gens]
-        codes = _compile_all(srcs, workers)
-        # the executor's register rule (staging and tail placement), so that what is timed is
-        # what would run
-        for j, (gen, code) in enumerate(zip(gens, codes)):
-            if F.kernel_registers(code).get("fused_kernel", 0) > F.MERGED_TAIL_MAX_VGPRS:
-                src = F.FusedExecutor._fit_registers(gen, srcs[j])
-                if src != srcs[j]:
-                    srcs[j], codes[j] = src, F.compile_source(src)[0]
+        srcs, codes = _fit_all(gens, workers)
+        if log:
+            log(f"group {si}: {len(gens)} candidates compiled")
         stats = group_stats(base.gen)
         sig = group_signature(st.group, graph)
         best = None
-        for gen, src, code in zip(gens, srcs, codes):
+        for j, (gen, src, code) in enumerate(zip(gens, srcs, codes)):
             cg = F.CompiledGroup(gen, src)
             cg.module = F.load_module(src)
             ln = ex.prepare_launch(cg, graph, results)
@@ -265,6 +287,8 @@ def autotune(graph, runtime, wanted: Sequence[str], seed: Optional[Dict[str, Any
                          "stats": stats, "features": features(stats, st.group.grid, gen.sched, gen.lds_bytes(), gen.fw)})
             if best is None or ms < best[0]:
                 best = (ms, gen.sched)
+            if log:
+                log(f"  {gen.sched.as_tuple()} {ms:.4f} ms ({j + 1}/{len(gens)})")
         if update_db and best is not None:
             db[sig] = list(best[1].as_tuple())
     if update_db:
@@ -525,7 +549,7 @@ def main(argv=None) -> int:
         g, sinks, *_ = compile_expressions(a.expr, boundary=a.boundary, field_map={"f": f, "u": u_})
         reds = [g.add(r, (sinks[0],), {}) for r in a.reductions.split(",") if r]
         rt = Runtime(device=dev, fusion_enabled=True)
-        rows = autotune(g, rt, sinks + reds)
+        rows = autotune(g, rt, sinks + reds, log=lambda m: print(m, flush=True))
         by = {}
         for r in rows:
             by.setdefault(r["sig"], []).append(r)

@@ -48,12 +48,19 @@ def oracle_streamed(expr, full, slab, r, boundary, dtype, dx=1.0):
 
 
 def check_stream_stats(st, o64, o32, what):
-    """The streamed reductions (L2, sum, mean, min, max) under the 2x rule."""
+    """The streamed reductions under the 2x rule.  L2, sum and mean accumulate, so their
+    scalars are compared directly (e_build <= 2 e_ref32 of the scalar).  Min and max select one
+    element: their error is that element's error, so they are held to twice the FP32 twin's
+    largest elementwise error over the field."""
     scalars = {"l2": lambda t: torch.linalg.vector_norm(t), "sum": lambda t: t.sum(),
-               "mean": lambda t: t.mean(), "min": lambda t: t.min(), "max": lambda t: t.max()}
+               "mean": lambda t: t.mean()}
     for k, fn in scalars.items():
         b = torch.tensor(float(st[k]), dtype=torch.float64)
         assert_2x(b, fn(o64).double(), fn(o32).double(), f"{what} {k}")
+    e_field = (o32.double() - o64.double()).abs().max().item()
+    for k, fn in (("min", torch.min), ("max", torch.max)):
+        e_b = abs(float(st[k]) - fn(o64).item())
+        assert e_b <= 2.0 * e_field, f"{what} {k}: {e_b:.3e} > 2 x field ref32 error {e_field:.3e}"
 
 
 @pytest.mark.parametrize("boundary", ["periodic", "neumann"])

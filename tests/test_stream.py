@@ -5,7 +5,8 @@ import torch
 
 import stenpy_amd as sp
 from stenpy_amd import gen
-from stenpy_amd.stream import GeneratorField, NpyLazyField, composite_radius, halo_ranges, stream_graph
+from stenpy_amd.stream import (GeneratorField, NpyLazyField, composite_radius, halo_ranges, plan_slabs,
+                               stream_graph)
 
 
 def ref_halo_rows(start, end, total, radius, boundary):
@@ -28,6 +29,30 @@ def test_halo_ranges_match_reference(bc):
             pieces, lp, rp = halo_ranges(start, end, total, r, bc)
             rows = [i for a, b in pieces for i in range(a, b)]
             assert (rows, lp, rp) == ref_halo_rows(start, end, total, r, bc)
+
+
+@pytest.mark.parametrize("bc", ["periodic", "neumann"])
+def test_slab_plans_rebuild_the_reference_halo_rows(bc):
+    """Every padded slab, assembled from its moves (host reads, copies from the previous slab's
+    device buffer, the saved first planes), holds exactly the rows the reference reads."""
+    for n0 in (1, 2, 5, 7, 16, 23, 40):
+        for slab in (1, 2, 3, 5, 7, 16, 40):
+            for r in (0, 1, 2, 3):
+                if bc == "periodic" and r > n0:
+                    continue
+                plans = plan_slabs(n0, slab, r, bc)
+                for pl in plans:
+                    rows = [None] * pl.planes
+                    for dst, kind, a, b in pl.moves:
+                        for i in range(b - a):
+                            if kind == "prev":
+                                u = plans[pl.k - 1].lo + a + i
+                                rows[dst + i] = u % n0 if bc == "periodic" else u
+                            else:                    # "host" and "wrap" name global planes
+                                rows[dst + i] = a + i
+                    want, lp, rp = ref_halo_rows(pl.s, pl.e, n0, r, bc)
+                    assert rows == want and pl.lpad == lp, (n0, slab, r, bc, pl.k)
+                    assert not [m for m in pl.moves if m[1] == "prev"] or pl.k > 0
 
 
 def test_composite_radius():
