@@ -184,6 +184,15 @@ int sp_reduce_combine(int kind, const double* records, int nrecords, int64_t cou
                       double* result, void* stream);
 int sp_reduce(int kind, const sp_tensor* x, double* result, void* workspace,
               const double* wparams, void* stream);
+/* sp_reduce without a workspace and without any allocation: the block records live in a
+ * file-scope device array of sp_reduce_noalloc_scratch_bytes() = 64 KiB per GPU (2048 records,
+ * the most the clamped stage-1 grid writes, so every input size fits; an input that needed
+ * more would be rejected with SP_EINVAL).  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU race. */
+int    sp_reduce_noalloc(int kind, const sp_tensor* x, double* result, const double* wparams,
+                         void* stream);
+size_t sp_reduce_noalloc_scratch_bytes(void);
 /* Many final combines in one launch (one block each): ``descs`` is a device array of n
  * descriptors, each as for sp_reduce_combine.  Used after a batched launch of many grids.
  * No counterpart in the reference (it reduces one tensor per call, engine 2340-2382). */

@@ -119,6 +119,8 @@ def _declare(lib: C.CDLL) -> None:
                          C.c_float, V, V], I),
         "sp_cast_f32_bf16": ([V, C.c_int64, V, V], I),
         "sp_probe_stream": ([I, I, V, V, V, V, C.c_int64, I, V], I),
+        "sp_reduce_noalloc": ([I, P, V, C.POINTER(D), V], I),
+        "sp_reduce_noalloc_scratch_bytes": ([], Z),
         "sp_probe_blocks": ([I, C.c_int64, I], C.c_int64),
     }
     for name, (args, res) in sig.items():
@@ -459,3 +461,19 @@ def probe_stream(mix: str, persistent: bool, a: torch.Tensor, u: Optional[torch.
 
 def probe_blocks(persistent: bool, n: int, n_cu: int = 256) -> int:
     return int(lib().sp_probe_blocks(int(persistent), n, n_cu))
+
+
+def reduce_noalloc(kind: str, x: torch.Tensor, out: torch.Tensor,
+                   wparams: Sequence[float] = None) -> torch.Tensor:
+    """``sp_reduce_noalloc``: the reduction of ``reduce`` into the caller's FP64 ``out`` (1
+    element, 2 for min_max), with the block records in the library's static device array; the
+    call allocates nothing."""
+    require_gpu(x, out)
+    if not x.is_contiguous():
+        raise ValueError("reduce_noalloc: the input must be contiguous (no copy is made)")
+    if out.dtype != torch.float64 or out.numel() < (2 if kind == "min_max" else 1):
+        raise ValueError("reduce_noalloc: out must hold FP64 results")
+    wp = (C.c_double * 4)(*(wparams or (1.0, 1.0, 1.0, 0.0)))
+    check(lib().sp_reduce_noalloc(REDUCE[kind], C.byref(desc(x)), out.data_ptr(), wp,
+                                  stream_ptr(x.device)), f"reduce_noalloc {kind}")
+    return out

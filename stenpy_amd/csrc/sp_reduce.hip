@@ -9,6 +9,8 @@
 // Stage 2: one block merges the records in FP64 in a fixed order.  Both stages have a fixed
 // reduction order, so results are bit-reproducible (the reference's GPU variance uses float
 // atomics and is not).
+#include <mutex>
+
 #include "sp_common.h"
 
 using namespace sp;
@@ -303,6 +305,57 @@ extern "C" int sp_reduce(int kind, const sp_tensor* x, double* result, void* wor
     hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(kCThreads), 0, st, kind,
                        (const double*)workspace, (int)blocks, n, result);
     return launch_status("sp_reduce(stage2)");
+}
+
+// ---- allocation-free variant ---------------------------------------------------------------
+// The block records of sp_reduce_noalloc live in this file-scope array instead of a caller
+// workspace: kMaxBlocks records of kRec doubles (64 KiB), held by every GPU the library runs
+// on until the process ends.  The stage-1 grid is clamped to kMaxBlocks, so any input size
+// fits, and every launched block writes all of its record before stage 2 reads it, so the
+// array needs no clearing between calls.  Calls that overlap on one GPU share it and race.
+static __device__ __align__(256) double scratch[kMaxBlocks * kRec];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of ``scratch`` on the current device, looked up once per device (host code
+// must not pass the array's name itself to a kernel: it is a host-side stand-in).
+int scratch_for_current_device(void** out) {
+    int dev = 0;
+    SP_HIP_CHECK(hipGetDevice(&dev));
+    SP_REQUIRE(dev >= 0 && dev < kMaxDevices, "reduce_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        SP_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        SP_REQUIRE(p != nullptr, "reduce_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SP_OK;
+}
+
+}  // namespace
+
+extern "C" size_t sp_reduce_noalloc_scratch_bytes(void) { return sizeof(scratch); }
+
+extern "C" int sp_reduce_noalloc(int kind, const sp_tensor* x, double* result,
+                                 const double* wparams, void* stream) {
+    void* ws = nullptr;
+    int rc = scratch_for_current_device(&ws);
+    if (rc) return rc;
+    SP_REQUIRE(x && x->dtype == SP_F32 && is_contiguous(x),
+               "reduce_noalloc: input must be contiguous f32");
+    // the records sp_reduce would need for this input, against the array's capacity
+    const int64_t n = numel(x) / (kind == SP_R_ANYSMALLDIAG ? 9 : 1);
+    int64_t blocks = (n + kThreads * 8 - 1) / (kThreads * 8);
+    if (blocks > kMaxBlocks) blocks = kMaxBlocks;
+    SP_REQUIRE((size_t)(blocks < 1 ? 1 : blocks) * kRec * sizeof(double) <= sizeof(scratch),
+               "reduce_noalloc: the input needs more block records than the scratch array holds");
+    return sp_reduce(kind, x, result, ws, wparams, stream);
 }
 
 extern "C" int sp_reduce_combine(int kind, const double* records, int nrecords, int64_t count,
