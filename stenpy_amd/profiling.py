@@ -95,3 +95,21 @@ def stats_markdown(d: str, command: str, top: int = 12) -> str:
         name = name if len(name) < 90 else name[:87] + "..."
         lines.append(f"| `{name}` | {r['calls']} | {r['avg_ns'] / 1e3:.2f} | {r['total_ns'] / 1e6:.3f} | {r['pct']:.1f} |")
     return "\n".join(lines) + "\n"
+
+
+def trace_tail(d: str, kernel: str, n: int) -> Dict[str, float]:
+    """Durations of the last ``n`` dispatches (by start time) of the kernel named ``kernel`` in
+    every ``*kernel_trace.csv`` under ``d``: the launches a timed loop ends with."""
+    rows = []
+    for p in _find(d, "kernel_trace.csv"):
+        with open(p, newline="") as fh:
+            for row in csv.DictReader(fh):
+                if (_col(row, "kernel_name") or "") == kernel:
+                    t0, t1 = float(_col(row, "start_timestamp")), float(_col(row, "end_timestamp"))
+                    rows.append((t0, t1 - t0, int(float(_col(row, "vgpr_count") or 0))))
+    rows.sort()
+    tail = [r[1] for r in rows[-n:]]
+    if not tail:
+        raise KeyError(f"no dispatch of {kernel!r} under {d}")
+    return {"dispatches": len(rows), "tail": len(tail), "mean_ns": sum(tail) / len(tail),
+            "min_ns": min(tail), "max_ns": max(tail), "vgprs": rows[-1][2]}

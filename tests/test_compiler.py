@@ -248,3 +248,39 @@ def test_random_pointwise_expressions_lower_to_their_values():
         np.testing.assert_allclose(got.numpy(), want, rtol=1e-12, atol=1e-12, err_msg=expr)
         checked += 1
     assert checked >= 40
+
+
+def test_fused_compiler_boundary_names():
+    """The reference's fused-path names: FUSIBLE_OPS covers the reference's set
+    (fused_compiler.py:194-195) and compile_and_execute rejects a non-HIP device with
+    ValueError (fused_compiler.py:634-636), which the reference's Runtime.run turns into its
+    eager fallback."""
+    from stenpy_amd import fused_compiler as fc
+    reference_set = {"add", "sub", "mul", "div", "exp", "log", "sqrt", "sin", "cos", "tanh", "neg",
+                     "laplacian", "gradient", "divergence", "curl", "hessian"}
+    assert reference_set <= fc.FUSIBLE_OPS
+    assert {"dot", "sqnorm", "eigenvalues", "determinant", "sum", "variance"} <= fc.FUSIBLE_OPS
+
+    class CpuRuntime:
+        device = torch.device("cpu")
+    g, sink, *_ = sp.compile_expression("laplacian(f)", field_map={"f": torch.zeros(4, 4, 4)})
+    with pytest.raises(ValueError, match="CUDA"):
+        fc.compile_and_execute(g, CpuRuntime(), None)
+
+
+def test_fused_compiler_reads_a_foreign_graph():
+    """compile_and_execute accepts any graph exposing the reference's node table."""
+    import collections
+    from stenpy_amd import fused_compiler as fc
+    Node = collections.namedtuple("Node", "id op_name input_ids params")
+
+    class ForeignGraph:
+        def __init__(self, nodes):
+            self._nodes = {n.id: n for n in nodes}
+    f = torch.zeros(4, 4, 4)
+    fg = ForeignGraph([Node("c", "_constant", (), {"value": f}),
+                       Node("lap", "laplacian", ("c",), {"dx": 1.0, "boundary": "periodic"}),
+                       Node("out", "exp", ("lap",), {})])
+    g = fc._as_graph(fg)
+    assert [n.id for n in g.topological_sort()] == ["c", "lap", "out"] and g.sinks() == ["out"]
+    assert g.node("lap").params == {"dx": 1.0, "boundary": "periodic"}

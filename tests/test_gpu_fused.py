@@ -196,3 +196,22 @@ def test_load_transformed_sources(bc, edge_mode):
     for expr in ["laplacian(tanh(f)) * tanh(f)", "divergence(gradient(tanh(f)))", "gradient(exp(f)) * 2",
                  "hessian(log(f*f+1))", "laplacian(laplacian(exp(f*0.5)))"]:
         run_and_check(expr, {"f": f}, boundary=bc, edge_mode=edge_mode)
+
+
+def test_reference_fused_entry_compile_and_execute():
+    """The call the reference's Runtime._run_fused makes (stenpy_engine.py:966-981): a graph
+    with FP64 field constants, a runtime exposing ``device``; the result holds the sink, widened
+    to FP64 for the reference's downstream asserts, within the 2x rule."""
+    from stenpy_amd import fused_compiler as fc
+    shape = (20, 22, 68)
+    f, u = field(shape, 2).double(), vfield(shape).double()
+    g, sink, _, fm, _ = sp.compile_expression(STENCIL_GRAPH, boundary="periodic", field_map={"f": f, "u": u})
+
+    class RefRuntime:
+        device = torch.device("cuda")
+    assert any(n.op_name in fc.FUSIBLE_OPS for n in g.nodes.values())
+    res = fc.compile_and_execute(g, RefRuntime(), None)
+    assert res[sink].dtype == torch.float64 and fm["f"] in res
+    o64 = graph_eval.evaluate(g, torch.float64)[sink]
+    o32 = graph_eval.evaluate(g, torch.float32)[sink]
+    assert_2x(res[sink], o64, o32, "compile_and_execute")
