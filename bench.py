@@ -119,7 +119,7 @@ def hbm_ceilings(f, u, reps: int = 10):
     n = f.numel()
     out = torch.empty_like(f)
     cases = {"copy": (8 * n, dict(a=f, out=out)), "4r1w": (20 * n, dict(a=f, u=u, out=out)),
-             "read": (12 * n, dict(a=u))}
+             "read": (12 * n, dict(a=u)), "write": (4 * n, dict(a=out))}
     res = {}
     for mix, (nbytes, kw) in cases.items():
         best = 0.0
@@ -157,15 +157,12 @@ def _line(metric, value, unit, steps, warmup, ms, config, **extra):
     print(json.dumps(d), flush=True)
 
 
-def bench_batched(args) -> int:
-    """96 fields, axes U{32..384} (seed 6), 6 random graphs of 3-12 ops (seed 6); field i uses
-    graph i mod 6 and seed 1000 + i; every level one persistent launch, replayed as a hipGraph."""
+def c3_items(dev):
+    """The batched workload: 96 fields with axes U{32..384} (seed 6) and 6 random graphs of
+    3-12 ops (seed 6); field i (seed 1000 + i) uses graph i mod 6.  Returns (exprs, items)."""
     import random
-    import numpy as np
-    import torch
     import stenpy_amd as sp
-    from stenpy_amd import batch as B, gen, randgraph, tune
-    dev = torch.device("cuda:0")
+    from stenpy_amd import gen, randgraph
     rng = random.Random(6)
     exprs = [randgraph.random_expression(rng, rng.randint(3, 12), vector_leaf=False, mid_mean=0.15)[0]
              for _ in range(6)]
@@ -175,6 +172,38 @@ def bench_batched(args) -> int:
         f = gen.generate(sh, 1000 + i, dev)
         g, sink, *_ = sp.compile_expression(exprs[i % 6], boundary="periodic", field_map={"f": f})
         items.append((g, [sink]))
+    return exprs, items
+
+
+TENSOR_EXPRS = ["eigenvalues(hessian(f))", "determinant(hessian(f))", "trace(hessian(f))",
+                "deviatoric(hessian(f))", "mean_curvature(f)"]
+
+
+def c5_graph(dev, n: int = 768):
+    """The tensor workload: n^3 FP32 field, lower half the seeded Fourier field (seed 8), upper
+    half i^2 + j^2 + k^2; one graph whose five sinks share the hessian.  Returns (g, sinks, f)."""
+    import torch
+    from stenpy_amd import gen
+    from stenpy_amd.compiler import compile_expressions
+    f = gen.generate((n, n, n), 8, dev)
+    i = torch.arange(n, device=dev, dtype=torch.float32)
+    q = (i.view(-1, 1, 1) ** 2 + i.view(1, -1, 1) ** 2 + i.view(1, 1, -1) ** 2)
+    f[n // 2:] = q[n // 2:]
+    del q
+    g, sinks, *_ = compile_expressions(TENSOR_EXPRS, boundary="periodic", field_map={"f": f})
+    return g, sinks, f
+
+
+def bench_batched(args) -> int:
+    """96 fields, axes U{32..384} (seed 6), 6 random graphs of 3-12 ops (seed 6); field i uses
+    graph i mod 6 and seed 1000 + i; every level one persistent launch, replayed as a hipGraph."""
+    import random
+    import numpy as np
+    import torch
+    import stenpy_amd as sp
+    from stenpy_amd import batch as B, gen, randgraph, tune
+    dev = torch.device("cuda:0")
+    exprs, items = c3_items(dev)
     t0 = time.perf_counter()
     br = B.BatchRunner(items, device=dev)
     br.compile(workers=8)
@@ -211,16 +240,8 @@ def bench_tensor(args) -> int:
     from stenpy_amd import gen, tune
     dev = torch.device("cuda:0")
     n = 768 if args.n == 512 else args.n
-    f = gen.generate((n, n, n), 8, dev)
-    i = torch.arange(n, device=dev, dtype=torch.float32)
-    q = (i.view(-1, 1, 1) ** 2 + i.view(1, -1, 1) ** 2 + i.view(1, 1, -1) ** 2)
-    f[n // 2:] = q[n // 2:]
-    del q
-    exprs = ["eigenvalues(hessian(f))", "determinant(hessian(f))", "trace(hessian(f))",
-             "deviatoric(hessian(f))", "mean_curvature(f)"]
-    from stenpy_amd.compiler import compile_expressions
-    # one graph, five sinks sharing the hessian: f is read once per evaluation
-    g, sinks, *_ = compile_expressions(exprs, boundary="periodic", field_map={"f": f})
+    exprs = TENSOR_EXPRS
+    g, sinks, f = c5_graph(dev, n)
     graphs = [(g, sinks)]
     rt = sp.Runtime(device=dev, fusion_enabled=True, schedule_fn=tune.schedule_fn(dev))
 
@@ -331,34 +352,62 @@ CALIBRATION_COPY = "k_unary_"
 
 
 def profile_pass(args) -> int:
-    """Fused stencil-graph launches only (plus one calibration copy): what rocprofv3 measures."""
+    """Only the measured kernels (plus one calibration copy): what rocprofv3 measures.
+    graph: the 512^3 stencil graph's fused group; batched: every level launch of the 96-grid
+    batch; tensor: the 768^3 five-sink group (determinant flag pass and main launch)."""
     import torch
     import stenpy_amd as sp
     from stenpy_amd import _lib as L, gen, tune
     dev = torch.device("cuda:0")
     n = args.n
-    f = gen.generate((n, n, n), 2, dev)
-    u = gen.generate_vector((n, n, n), (3, 4, 5), dev)
-    g, sink, reds = build_graph(f, u)
-    rt = sp.Runtime(device=dev, fusion_enabled=True, schedule_fn=tune.schedule_fn(dev))
-    res = rt.fused.run(g, rt, wanted=[sink] + reds)
-    cg, ln = group_launch(rt, g, res, sink)
-    for _ in range(args.steps):
-        ln.launch()
-    out = torch.empty_like(f)
+    info = {"profile_pass": True, "workload": args.workload, "launches": args.steps,
+            "copy_bytes": copy_bytes(n)}
+    if args.workload == "batched":
+        from stenpy_amd import batch as B
+        _, items = c3_items(dev)
+        br = B.BatchRunner(items, device=dev)
+        br.compile(workers=8)
+        for _ in range(args.steps):
+            br.run()
+        info["stats"] = {k: v for k, v in br.stats().items() if isinstance(v, (int, float))}
+    elif args.workload == "tensor":
+        g, sinks, _ = c5_graph(dev, 768)
+        rt = sp.Runtime(device=dev, fusion_enabled=True, schedule_fn=tune.schedule_fn(dev))
+        res = rt.fused.run(g, rt, wanted=sinks)
+        cg, ln = group_launch(rt, g, res, sinks[0])
+        for _ in range(args.steps):
+            ln.run_once()
+        from stenpy_amd import fused as F
+        info.update(schedule=list(cg.gen.sched.as_tuple()), grid=ln.nblocks + cg.gen.boundary_blocks(),
+                    split_tail=cg.gen.split_tail, staged=bool(cg.gen.stage),
+                    staged_in_source="so_stage" in cg.source,
+                    vgprs=F.kernel_registers(F.compile_source(cg.source)[0]))
+    else:
+        f = gen.generate((n, n, n), 2, dev)
+        u = gen.generate_vector((n, n, n), (3, 4, 5), dev)
+        g, sink, reds = build_graph(f, u)
+        rt = sp.Runtime(device=dev, fusion_enabled=True, schedule_fn=tune.schedule_fn(dev))
+        res = rt.fused.run(g, rt, wanted=[sink] + reds)
+        cg, ln = group_launch(rt, g, res, sink)
+        for _ in range(args.steps):
+            ln.launch()
+        info.update(schedule=list(cg.gen.sched.as_tuple()), grid=ln.nblocks + cg.gen.boundary_blocks())
+    # a kernel of known arithmetic for the flop counters: 2^18 threads, 12 FP32 adds each
+    pf = torch.ones(1 << 20, device=dev)
+    L.probe_stream("4r1w", False, a=pf, u=torch.ones(3 << 20, device=dev), out=torch.empty_like(pf))
+    c = gen.generate((n, n, n), 2, dev)
+    out = torch.empty_like(c)
     for _ in range(3):
-        L.unary("neg", f, out)          # streaming copy: reads and writes copy_bytes(n)
+        L.unary("neg", c, out)          # streaming copy: reads and writes copy_bytes(n)
     torch.cuda.synchronize()
-    print(json.dumps({"profile_pass": True, "schedule": list(cg.gen.sched.as_tuple()),
-                      "launches": args.steps, "grid": ln.nblocks + cg.gen.boundary_blocks(),
-                      "copy_bytes": copy_bytes(n)}), flush=True)
+    print(json.dumps(info), flush=True)
     return 0
 
 
 def offline_reports(args) -> int:
     from stenpy_amd import profiling
     if args.pmc_traffic:
-        t = profiling.traffic(args.pmc_traffic[0], args.pmc_traffic[1], kernel="fused_kernel",
+        t = profiling.traffic(args.pmc_traffic[0], args.pmc_traffic[1], kernel=args.kernel,
                               copy_kernel=CALIBRATION_COPY, copy_bytes=copy_bytes(args.n))
         try:        # the edge-band tail kernel of the same group, when it runs separately
             tt = profiling.traffic(args.pmc_traffic[0], args.pmc_traffic[1], kernel="fused_tail",
@@ -368,7 +417,8 @@ def offline_reports(args) -> int:
             t["bytes_per_launch"] = t["main_bytes"] + t["tail_bytes"]
         except KeyError:
             pass
-        t["algorithmic_bytes"] = 20 * args.n ** 3
+        t["kernel"] = args.kernel
+        t["algorithmic_bytes"] = args.alg_bytes or 20 * args.n ** 3
         t["n"] = args.n
         # the schedule the profile pass ran (its own JSON line in the fetch pass's log)
         for log in sorted(Path(args.pmc_traffic[0]).parent.glob(Path(args.pmc_traffic[0]).name + "*.log")):
@@ -408,6 +458,10 @@ def main() -> int:
     ap.add_argument("--stats-md", type=str, default=None, metavar="DIR",
                     help="render a rocprofv3 --stats CSV output as markdown (no GPU)")
     ap.add_argument("--cmd", type=str, default="", help="command line quoted in --stats-md output")
+    ap.add_argument("--kernel", type=str, default="fused_kernel",
+                    help="with --pmc-traffic: the kernel whose bytes per launch are reported")
+    ap.add_argument("--alg-bytes", type=int, default=0,
+                    help="with --pmc-traffic: algorithmic bytes per launch (default 20 n^3)")
     ap.add_argument("--tail", type=int, default=0,
                     help="with --stats-md: also report the mean of the last N fused_kernel dispatches")
     ap.add_argument("--out", type=str, default=None)
@@ -499,7 +553,7 @@ def main() -> int:
                    "eager_launch_gvoxel_s": round(nvox / t_fused_eager / 1e9, 2),
                    "eager_launch_ms_per_step": round(t_fused_eager * 1e3, 4),
                    "probe_gbs": {"copy_1r1w": ceil["copy"], "stencil_mix_4r1w": ceil["4r1w"],
-                                 "read_only": ceil["read"]},
+                                 "read_only": ceil["read"], "write_only": ceil["write"]},
                    "bf16_out_hbm_frac_step": round(18 * nvox / t_bf16 / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

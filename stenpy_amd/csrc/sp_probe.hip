@@ -8,6 +8,7 @@
 //                   so that every load is fully coalesced: the probe measures the mix, not an
 //                   access pattern
 //   SP_PROBE_READ   per-block sums of a[i]                 reads only (one double per block)
+//   SP_PROBE_WRITE  out[i] = i-dependent constant           writes only
 //
 // Each mix has two launch shapes: one float4 group per thread over the whole range ("flat"),
 // and a grid-stride loop over 8 blocks per CU, one group per iteration ("persistent").  The
@@ -67,6 +68,15 @@ __global__ void __launch_bounds__(kThreads) k_read(const float4* __restrict__ a,
     }
 }
 
+__global__ void __launch_bounds__(kThreads) k_write(float4* __restrict__ o, int64_t n4,
+                                                    int persistent) {
+    const int64_t stride = persistent ? (int64_t)gridDim.x * kThreads : n4;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += stride) {
+        const float v = (float)(i & 1023);
+        o[i] = make_float4(v, v + 1.f, v + 2.f, v + 3.f);
+    }
+}
+
 }  // namespace
 
 using namespace sp;
@@ -95,6 +105,10 @@ extern "C" int sp_probe_stream(int mix, int persistent, const float* a, const fl
                        "probe: u and out must be 16-byte aligned");
             hipLaunchKernelGGL(k_4r1w, dim3(g), dim3(kThreads), 0, s, (const float4*)a,
                                (const float4*)u, (float4*)out, n4, persistent);
+            break;
+        case SP_PROBE_WRITE:
+            SP_REQUIRE(((uintptr_t)a & 15) == 0, "probe: out must be 16-byte aligned");
+            hipLaunchKernelGGL(k_write, dim3(g), dim3(kThreads), 0, s, (float4*)a, n4, persistent);
             break;
         case SP_PROBE_READ:
             SP_REQUIRE(partials, "probe: partials (one double per block) required");
