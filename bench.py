@@ -509,14 +509,16 @@ def main() -> int:
     def plain():
         rt.run_eager(g, outputs=wanted)
 
-    t_fused_eager = time_steps(fused_fp32, args.steps, args.warmup)
-    t_bf16_eager = time_steps(fused_bf16, args.steps, args.warmup)
-    # the same evaluation replayed from a hipGraph (no per-step host work)
+    # the same evaluation replayed from a hipGraph (no per-step host work); on this runtime a
+    # replay costs more per step than launching the two kernels directly, so the step timed
+    # for the headline is the direct launch, measured last (plans compiled, clocks settled)
     cap, _ = rt.fused.capture(g, rt, wanted)
     cap16, _ = rt.fused.capture(g, rt, wanted, out_dtype=torch.bfloat16)
-    t_fused = time_steps(cap.replay, args.steps, args.warmup)
-    t_bf16 = time_steps(cap16.replay, args.steps, args.warmup)
+    t_replay = time_steps(cap.replay, args.steps, args.warmup)
+    t_bf16_replay = time_steps(cap16.replay, args.steps, args.warmup)
     t_plain = time_steps(plain, max(3, args.steps // 4), 1)
+    t_bf16 = time_steps(fused_bf16, args.steps, args.warmup)
+    t_fused = time_steps(fused_fp32, args.steps, args.warmup)
 
     cg, ln = group_launch(rt, g, res, sink)
     # average duration of the group's kernels over the timed launches (HIP events on their
@@ -546,12 +548,13 @@ def main() -> int:
                    "plain_gvoxel_s": round(nvox / t_plain / 1e9, 3),
                    "plain_ms_per_step": round(t_plain * 1e3, 3),
                    "speedup_vs_plain": round(t_plain / t_fused, 2),
-                   "speedup_vs_plain_both_eager_launch": round(t_plain / t_fused_eager, 2),
                    "bf16_out_gvoxel_s": round(nvox / t_bf16 / 1e9, 2),
                    "schedule_source": schedule_source(cg, g),
-                   "step": "hipGraph replay of the fused evaluation (fused kernels, 4 FP64 combines)",
-                   "eager_launch_gvoxel_s": round(nvox / t_fused_eager / 1e9, 2),
-                   "eager_launch_ms_per_step": round(t_fused_eager * 1e3, 4),
+                   "step": "the runtime's fused evaluation, launched directly: fused_kernel, then one "
+                           "launch of the 4 FP64 final combines; output buffers from torch's caching allocator",
+                   "graph_replay_gvoxel_s": round(nvox / t_replay / 1e9, 2),
+                   "graph_replay_ms_per_step": round(t_replay * 1e3, 4),
+                   "bf16_graph_replay_gvoxel_s": round(nvox / t_bf16_replay / 1e9, 2),
                    "probe_gbs": {"copy_1r1w": ceil["copy"], "stencil_mix_4r1w": ceil["4r1w"],
                                  "read_only": ceil["read"], "write_only": ceil["write"]},
                    "bf16_out_hbm_frac_step": round(18 * nvox / t_bf16 / 1e9 / HBM_PEAK_GBS, 4)},

@@ -11,6 +11,19 @@
 
 __device__ __forceinline__ float sym3_trace(const float* a) { return (a[0] + a[4]) + a[8]; }
 
+// x / 3, correctly rounded, without the division sequence: q = x r with r = RN(1/3), then one
+// FMA correction q + RN(x - 3q) r (Markstein's scheme for a divisor known in advance).  Checked
+// over every finite FP32 x against the FP64 quotient rounded to FP32 (x / 3 is never within
+// 2^-51 of an FP32 rounding midpoint, so that rounding is the correct one); the sign is x's
+// (the correction turns -0 into +0), infinities and NaN return the product x r.  Four
+// instructions instead of about a dozen.
+__device__ __forceinline__ float div3f(float x) {
+    const float r = 0.3333333432674407958984375f;   // RN(1/3)
+    const float q = x * r;
+    const float q2 = fmaf(fmaf(-q, 3.0f, x), r, q);
+    return q2 == q2 ? copysignf(q2, x) : q;
+}
+
 // a*d - b*c with one rounding error (Kahan's FMA scheme).
 __device__ __forceinline__ float det2_fma(float a, float b, float c, float d) {
     const float w = b * c;
@@ -55,15 +68,16 @@ __device__ __forceinline__ void sort3(float* e) {
 //     D = m3^2 + 2 sum_{i<j, o} ((b_j - b_i) c_o - (c_j - c_i) b_o)^2 + 12 sum_{o<o'} (b_o c_o' - c_o b_o')^2
 // with b = B, c = B^2, i, j diagonal and o, o' off-diagonal positions.  Each minor carries an
 // absolute error of a few ulp of |B|^3, so the angle is accurate to a few ulp for every
-// spectrum, double and triple eigenvalues included.  B is scaled by its largest entry first,
-// which keeps |B|^6 inside the FP32 range.  A diagonal matrix returns its sorted diagonal exactly.
+// spectrum, double and triple eigenvalues included.  B is first scaled exactly, by a power of
+// two that brings its largest entry into [1, 2), which keeps |B|^6 inside the FP32 range.  A
+// diagonal matrix returns its sorted diagonal exactly.
 __device__ __forceinline__ void sym3_eigvalsh(const float* m, float* e) {
     if (m[3] == 0.0f && m[6] == 0.0f && m[7] == 0.0f) {
         e[0] = m[0]; e[1] = m[4]; e[2] = m[8];
         sort3(e);
         return;
     }
-    const float q = ((m[0] + m[4]) + m[8]) / 3.0f;
+    const float q = div3f((m[0] + m[4]) + m[8]);
     float b0 = m[0] - q, b1 = m[4] - q, b2 = m[8] - q, o01 = m[3], o02 = m[6], o12 = m[7];
     const float mx = fmaxf(fmaxf(fmaxf(fabsf(b0), fabsf(b1)), fmaxf(fabsf(b2), fabsf(o01))),
                            fmaxf(fabsf(o02), fabsf(o12)));
@@ -72,8 +86,11 @@ __device__ __forceinline__ void sym3_eigvalsh(const float* m, float* e) {
         e[0] = e[1] = e[2] = q + 0.0f * ((o01 + o02) + o12);
         return;
     }
-    const float s = 1.0f / mx;
-    b0 *= s; b1 *= s; b2 *= s; o01 *= s; o02 *= s; o12 *= s;
+    // exact scaling by a power of two: the largest entry lands in [1, 2)
+    int k;
+    (void)frexpf(mx, &k);
+    b0 = ldexpf(b0, 1 - k); b1 = ldexpf(b1, 1 - k); b2 = ldexpf(b2, 1 - k);
+    o01 = ldexpf(o01, 1 - k); o02 = ldexpf(o02, 1 - k); o12 = ldexpf(o12, 1 - k);
     // C = B^2
     const float c0 = b0 * b0 + o01 * o01 + o02 * o02;
     const float c1 = o01 * o01 + b1 * b1 + o12 * o12;
@@ -98,8 +115,9 @@ __device__ __forceinline__ void sym3_eigvalsh(const float* m, float* e) {
     const float disc = fmaf(m3, m3, fmaf(2.0f, s2, 12.0f * s1));
     const float detb = (b0 * (b1 * b2 - o12 * o12) - o01 * (o01 * b2 - o12 * o02)) +
                        o02 * (o01 * o12 - b1 * o02);
-    const float phi = atan2f(sqrtf(disc), 5.19615242270663188f * detb) / 3.0f;
-    const float p2 = 2.0f * mx * sqrtf(((c0 + c1) + c2) / 6.0f);
+    const float phi = div3f(atan2f(sqrtf(disc), 5.19615242270663188f * detb));
+    // (c0 + c1 + c2) / 6 as div3f(.) / 2: exact halving, the sum being >= 1 after the scaling
+    const float p2 = ldexpf(2.0f * sqrtf(div3f((c0 + c1) + c2) * 0.5f), k - 1);
     float sn, cs;
     sincosf(phi, &sn, &cs);
     e[2] = fmaf(p2, cs, q);
@@ -114,7 +132,7 @@ __device__ __forceinline__ float finite_or_zero(float x) {
 }
 
 __device__ __forceinline__ void sym3_deviatoric(const float* m, float* d) {
-    const float t = sym3_trace(m) / 3.0f;
+    const float t = div3f(sym3_trace(m));
 #pragma unroll
     for (int i = 0; i < 9; ++i) d[i] = finite_or_zero((i % 4 == 0) ? m[i] - t : m[i]);
 }

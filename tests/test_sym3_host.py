@@ -1,0 +1,90 @@
+"""The per-voxel 3x3 routines of stenpy_amd/csrc/sp_sym3.h built as plain C++ for the host and
+checked on the CPU: ``div3f`` (x / 3 by multiplication and one FMA correction) against the FP64
+quotient rounded to FP32, over whole binades; ``sym3_eigvalsh`` against numpy's FP64 eigvalsh on
+random, nearly degenerate and widely scaled symmetric matrices."""
+import ctypes
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+HDR = ROOT / "stenpy_amd" / "csrc" / "sp_sym3.h"
+
+_SHIM = r"""
+#include <cmath>
+#define __device__
+#define __forceinline__ inline
+#include "sp_sym3.h"
+extern "C" void div3(const float* x, float* y, long n) { for (long i = 0; i < n; ++i) y[i] = div3f(x[i]); }
+extern "C" void eig(const float* m, float* e, long n) { for (long i = 0; i < n; ++i) sym3_eigvalsh(m + 9 * i, e + 3 * i); }
+extern "C" void dev(const float* m, float* d, long n) { for (long i = 0; i < n; ++i) sym3_deviatoric(m + 9 * i, d + 9 * i); }
+"""
+
+
+@pytest.fixture(scope="module")
+def sym3(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("no host C++ compiler")
+    d = tmp_path_factory.mktemp("sym3")
+    (d / "shim.cpp").write_text(_SHIM)
+    so = d / "libsym3.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", f"-I{HDR.parent}",
+                    "-o", str(so), str(d / "shim.cpp")], check=True)
+    lib = ctypes.CDLL(str(so))
+    for fn in (lib.div3, lib.eig, lib.dev):
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
+    return lib
+
+
+def _div3(lib, x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    lib.div3(x.ctypes.data, y.ctypes.data, x.size)
+    return y
+
+
+def test_div3f_is_correctly_rounded(sym3):
+    """Normal inputs with normal quotients: the scheme commutes with scaling by powers of two,
+    so the binade [1, 2) covers them; the binades where x / 3 is subnormal or x is the largest
+    are checked whole, with both signs.  The reference rounding is FP64 x / 3 rounded to FP32,
+    which is the correct rounding (x / 3 is never within 2^-51 of an FP32 midpoint)."""
+    def binade(e):
+        bits = (np.arange(1 << 23, dtype=np.uint32) | np.uint32((e + 127) << 23)) if e > -127 else \
+            np.arange(1 << 23, dtype=np.uint32)
+        return bits.view(np.float32)
+    for e in (0, -126, -127, -125, 127, 126):
+        x = binade(e)
+        x = np.concatenate([x, -x])
+        got = _div3(sym3, x)
+        want = (x.astype(np.float64) / 3.0).astype(np.float32)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), e
+    sp = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 3.0, -3.0, 1.5, 3.4028235e38], dtype=np.float32)
+    got = _div3(sym3, sp)
+    want = (sp.astype(np.float64) / 3.0).astype(np.float32)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    assert np.array_equal(got[ok].view(np.uint32), want[ok].view(np.uint32))
+
+
+def test_eigvalsh_against_numpy_fp64(sym3):
+    """Eigenvalues within a few FP32 ulp of the spectral radius (the bound of the closed form's
+    angle), on well separated, nearly double and triple, and widely scaled spectra."""
+    rng = np.random.default_rng(3)
+    mats = []
+    for scale in (1e-30, 1e-3, 1.0, 1e3, 1e25):
+        a = rng.standard_normal((2000, 3, 3))
+        mats.append(scale * (a + a.transpose(0, 2, 1)))
+        q, _ = np.linalg.qr(rng.standard_normal((2000, 3, 3)))
+        lam = np.stack([np.ones(2000), 1 + 1e-6 * rng.standard_normal(2000), 1 + 1e-3 * rng.standard_normal(2000)], -1)
+        mats.append(scale * (q * lam[:, None, :]) @ q.transpose(0, 2, 1))
+    m = np.concatenate(mats).astype(np.float32)
+    m = 0.5 * (m + m.transpose(0, 2, 1))
+    e = np.empty((m.shape[0], 3), dtype=np.float32)
+    sym3.eig(np.ascontiguousarray(m).ctypes.data, e.ctypes.data, m.shape[0])
+    ref = np.linalg.eigvalsh(m.astype(np.float64))
+    rad = np.abs(ref).max(axis=1, keepdims=True)
+    err = np.abs(e.astype(np.float64) - ref) / (rad * 2.0 ** -24)
+    assert err.max() <= 16.0, err.max()
