@@ -87,4 +87,5 @@ def test_eigvalsh_against_numpy_fp64(sym3):
     ref = np.linalg.eigvalsh(m.astype(np.float64))
     rad = np.abs(ref).max(axis=1, keepdims=True)
     err = np.abs(e.astype(np.float64) - ref) / (rad * 2.0 ** -24)
+    print("eig max err (ulp of radius)", err.max())
     assert err.max() <= 16.0, err.max()
