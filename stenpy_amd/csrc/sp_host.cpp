@@ -24,16 +24,44 @@ inline uint64_t splitmix_at(uint64_t seed, uint64_t lin) {
     return z ^ (z >> 31);
 }
 
-__attribute__((target_clones("avx512f", "avx2", "default")))
-void row_kernel(float* __restrict__ out, int64_t n2, const double* __restrict__ cr,
-                const double* __restrict__ ci, const double* __restrict__ C,
-                const double* __restrict__ S, uint64_t seed, uint64_t lin0) {
-    for (int64_t k = 0; k < n2; ++k) {
-        double acc = 0.0;
-        for (int m = 0; m < 8; ++m) acc += cr[m] * C[m * n2 + k] - ci[m] * S[m * n2 + k];
-        const double u = (double)(splitmix_at(seed, lin0 + (uint64_t)k) >> 11) * (1.0 / 9007199254740992.0);
-        out[k] = (float)(acc + (2.0 * u - 1.0) * 1e-3);
+// The row loop vectorises over k once 64-bit multiplies and u64 -> double conversions are
+// vector instructions (AVX-512 DQ); AVX2 and plain builds keep the scalar noise term.
+#define SP_ROW_BODY                                                                                  \
+    for (int64_t k = 0; k < n2; ++k) {                                                               \
+        double acc = 0.0;                                                                            \
+        for (int m = 0; m < 8; ++m) acc += cr[m] * C[m * n2 + k] - ci[m] * S[m * n2 + k];           \
+        const double u = (double)(splitmix_at(seed, lin0 + (uint64_t)k) >> 11) * (1.0 / 9007199254740992.0); \
+        out[k] = (float)(acc + (2.0 * u - 1.0) * 1e-3);                                              \
     }
+
+using RowFn = void (*)(float*, int64_t, const double*, const double*, const double*, const double*,
+                       uint64_t, uint64_t);
+
+__attribute__((target("avx512f,avx512dq,avx512vl,avx512bw,fma")))
+void row_avx512dq(float* __restrict__ out, int64_t n2, const double* __restrict__ cr,
+                  const double* __restrict__ ci, const double* __restrict__ C,
+                  const double* __restrict__ S, uint64_t seed, uint64_t lin0) {
+    SP_ROW_BODY
+}
+
+__attribute__((target("avx2,fma")))
+void row_avx2(float* __restrict__ out, int64_t n2, const double* __restrict__ cr,
+              const double* __restrict__ ci, const double* __restrict__ C,
+              const double* __restrict__ S, uint64_t seed, uint64_t lin0) {
+    SP_ROW_BODY
+}
+
+void row_plain(float* __restrict__ out, int64_t n2, const double* __restrict__ cr,
+               const double* __restrict__ ci, const double* __restrict__ C,
+               const double* __restrict__ S, uint64_t seed, uint64_t lin0) {
+    SP_ROW_BODY
+}
+
+RowFn pick_row() {
+    __builtin_cpu_init();
+    if (__builtin_cpu_supports("avx512dq") && __builtin_cpu_supports("avx512vl")) return row_avx512dq;
+    if (__builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma")) return row_avx2;
+    return row_plain;
 }
 
 }  // namespace
@@ -52,6 +80,7 @@ extern "C" int sp_host_gen_fourier(float* out, const int64_t n[3], int64_t z0, i
             S[m * n2 + k] = std::sin(t);
         }
     const int64_t rows = nz * n1;
+    static const RowFn row = pick_row();
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel for schedule(static)
     for (int64_t r = 0; r < rows; ++r) {
@@ -63,7 +92,7 @@ extern "C" int sp_host_gen_fourier(float* out, const int64_t n[3], int64_t z0, i
             cr[m] = a[m] * std::cos(t);
             ci[m] = a[m] * std::sin(t);
         }
-        row_kernel(out + r * n2, n2, cr, ci, C.data(), S.data(), seed,
+        row(out + r * n2, n2, cr, ci, C.data(), S.data(), seed,
                    (uint64_t)((i * n1 + j) * n2));
     }
     return 0;
