@@ -52,9 +52,37 @@ __device__ __forceinline__ float sym3_det_regularised(const float* m, float reg)
 // of a few ulp of |B|.
 #if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC_RTC__)
 #define sym3_sqrt(x) __builtin_amdgcn_sqrtf(x)
+#define sym3_rcp(x) __builtin_amdgcn_rcpf(x)
 #else
 #define sym3_sqrt(x) sqrtf(x)
+#define sym3_rcp(x) (1.0f / (x))
 #endif
+
+// atan2(y, x) for y >= 0 (or NaN), the closed form's angle in [0, pi]: a = min / max of |x| and
+// y (the hardware reciprocal and one FMA correction), atan a = a + a^3 P(a^2) on [0, 1] with P of
+// degree 8 fitted like the sine and cosine below (0.94 ulp in FP32), then pi/2 - r above the
+// diagonal and pi - r for x < 0 (-0 included, as atan2).  NaN in either argument gives NaN.
+__device__ __forceinline__ float atan2_upper(float y, float x) {
+    const float ax = fabsf(x);
+    const bool sw = y > ax;
+    const float lo = sw ? ax : y, hi = sw ? y : ax;
+    const float rc = sym3_rcp(hi);
+    float a = lo * rc;
+    a = fmaf(fmaf(-a, hi, lo), rc, a);
+    a = hi == 0.0f ? 0.0f : a;                    // atan2(0, +-0): a = 0
+    const float t = a * a;
+    float p = fmaf(t, -1.789061236e-03f, 1.089788415e-02f);
+    p = fmaf(p, t, -3.115503304e-02f);
+    p = fmaf(p, t, 5.794504657e-02f);
+    p = fmaf(p, t, -8.403480053e-02f);
+    p = fmaf(p, t, 1.095253304e-01f);
+    p = fmaf(p, t, -1.426439285e-01f);
+    p = fmaf(p, t, 1.999857426e-01f);
+    p = fmaf(p, t, -3.333330154e-01f);
+    float r = fmaf(a * t, p, a);
+    r = sw ? 1.57079637050628662f - r : r;
+    return __builtin_signbit(x) ? 3.14159274101257324f - r : r;
+}
 
 // sin and cos of phi in [0, pi/3] (the closed form's angle), without range reduction:
 // sin = x + x^3 S(x^2), cos = 1 - x^2/2 + x^4 C(x^2), S and C of degree 4 fitted by weighted
@@ -142,7 +170,7 @@ __device__ __forceinline__ void sym3_eigvalsh(const float* m, float* e) {
     const float disc = fmaf(m3, m3, fmaf(2.0f, s2, 12.0f * s1));
     const float detb = (b0 * (b1 * b2 - o12 * o12) - o01 * (o01 * b2 - o12 * o02)) +
                        o02 * (o01 * o12 - b1 * o02);
-    const float phi = div3f(atan2f(sym3_sqrt(disc), 5.19615242270663188f * detb));
+    const float phi = div3f(atan2_upper(sym3_sqrt(disc), 5.19615242270663188f * detb));
     // (c0 + c1 + c2) / 6 as div3f(.) / 2: exact halving, the sum being >= 1 after the scaling
     const float p2 = ldexpf(2.0f * sym3_sqrt(div3f((c0 + c1) + c2) * 0.5f), k - 1);
     float sn, cs;

@@ -1,7 +1,8 @@
 """The per-voxel 3x3 routines of stenpy_amd/csrc/sp_sym3.h built as plain C++ for the host and
 checked on the CPU: ``div3f`` (x / 3 by multiplication and one FMA correction) against the FP64
-quotient rounded to FP32, over whole binades; ``sym3_eigvalsh`` against numpy's FP64 eigvalsh on
-random, nearly degenerate and widely scaled symmetric matrices."""
+quotient rounded to FP32, over whole binades; ``atan2_upper`` against numpy's FP64 arctan2;
+``sym3_eigvalsh`` against numpy's FP64 eigvalsh on random, nearly degenerate and widely scaled
+symmetric matrices."""
 import ctypes
 import shutil
 import subprocess
@@ -21,6 +22,7 @@ _SHIM = r"""
 extern "C" void div3(const float* x, float* y, long n) { for (long i = 0; i < n; ++i) y[i] = div3f(x[i]); }
 extern "C" void eig(const float* m, float* e, long n) { for (long i = 0; i < n; ++i) sym3_eigvalsh(m + 9 * i, e + 3 * i); }
 extern "C" void dev(const float* m, float* d, long n) { for (long i = 0; i < n; ++i) sym3_deviatoric(m + 9 * i, d + 9 * i); }
+extern "C" void atan2u(const float* y, const float* x, float* r, long n) { for (long i = 0; i < n; ++i) r[i] = atan2_upper(y[i], x[i]); }
 """
 
 
@@ -36,6 +38,7 @@ def sym3(tmp_path_factory):
     lib = ctypes.CDLL(str(so))
     for fn in (lib.div3, lib.eig, lib.dev):
         fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
+    lib.atan2u.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
     return lib
 
 
@@ -89,3 +92,23 @@ def test_eigvalsh_against_numpy_fp64(sym3):
     err = np.abs(e.astype(np.float64) - ref) / (rad * 2.0 ** -24)
     print("eig max err (ulp of radius)", err.max())
     assert err.max() <= 16.0, err.max()
+
+
+def test_atan2_upper_against_numpy(sym3):
+    """The closed form's angle: atan2(y, x) for y >= 0 within 2 ulp of pi over a grid of both
+    signs of x, magnitudes 2^-60 .. 2^60, and the axes and zeros."""
+    rng = np.random.default_rng(5)
+    y = np.abs(rng.standard_normal(200000) * 2.0 ** rng.integers(-60, 60, 200000)).astype(np.float32)
+    x = (rng.standard_normal(200000) * 2.0 ** rng.integers(-60, 60, 200000)).astype(np.float32)
+    special_y = np.array([0, 0, 0, 1, 1, 1, 1e-30, 3e38, 1, 0], dtype=np.float32)
+    special_x = np.array([0, -0.0, 1, 0, -0.0, -1, -1e30, 1, 1, -5], dtype=np.float32)
+    y, x = np.concatenate([y, special_y, y[:1000]]), np.concatenate([x, special_x, y[:1000]])
+    got = np.empty_like(y)
+    sym3.atan2u(y.ctypes.data, x.ctypes.data, got.ctypes.data, y.size)
+    ref = np.arctan2(y.astype(np.float64), x.astype(np.float64))
+    err = np.abs(got - ref) / (np.pi * 2.0 ** -24)
+    assert err.max() <= 2.0, (err.max(), y[err.argmax()], x[err.argmax()])
+    nan = np.array([np.nan, 1.0], dtype=np.float32), np.array([1.0, np.nan], dtype=np.float32)
+    out = np.empty(2, dtype=np.float32)
+    sym3.atan2u(nan[0].ctypes.data, nan[1].ctypes.data, out.ctypes.data, 2)
+    assert np.isnan(out).all()
