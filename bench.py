@@ -251,14 +251,30 @@ def bench_tensor(args) -> int:
     t = time_steps(step, max(3, args.steps // 4), 1)
     nvox = n ** 3
     alg = nvox * 4 * (1 + 3 + 1 + 1 + 9 + 1)     # read f once, write 15 output channels (64 B)
+    res = rt.fused.run(g, rt, wanted=sinks)
+    cg, ln = group_launch(rt, g, res, sinks[0])
+    # the group's kernels per evaluation (flag pass, main z-march, edge band), HIP events
+    k_ms = ln.time_ms(reps=max(3, args.steps // 4), warmup=1, rounds=1)
+    achieved = alg / (k_ms * 1e-3) / 1e9
+    traffic = None
+    tp = ROOT / "profiles" / "traffic_tensor_768.json"
+    if tp.exists():
+        tj = json.loads(tp.read_text())
+        if tj.get("n") == n and tuple(tj.get("schedule", ())) == tuple(cg.gen.sched.as_tuple()):
+            traffic = round(tj["bytes_per_launch"])
     _line("tensor ops: Gvoxel/s of hessian -> eig/det/trace/dev + mean curvature on 768^3",
           round(nvox / t / 1e9, 3), "Gvoxel/s", max(3, args.steps // 4), 1, t * 1e3,
           {"workload": f"{n}^3 FP32, lower half Fourier seed 8, upper half i^2+j^2+k^2; periodic; "
                        "one graph with five sinks sharing the hessian", "exprs": exprs,
-           "groups": len(rt.fused._plans and next(iter(rt.fused._plans.values()))[2])},
-          roofline={"bound": "hbm", "achieved": round(alg / t / 1e9, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(alg / t / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                    "algorithmic_bytes_per_launch": alg})
+           "groups": len(rt.fused._plans and next(iter(rt.fused._plans.values()))[2]),
+           "schedule": list(cg.gen.sched.as_tuple()), "split_tail": cg.gen.split_tail,
+           "staged_wide_stores": bool(cg.gen.stage)},
+          roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel_ms": round(k_ms, 4), "algorithmic_bytes_per_launch": alg,
+                    "kernels": "fused_flag (+ fused_flag_tail), fused_kernel, fused_tail (side stream)",
+                    "note": "VALU issue is the co-limit: counters give ~68% of the kernel's cycles "
+                            "issuing VALU (profiles/tensor_768_fused_kernel_counters.md)"})
     return 0
 
 
@@ -419,7 +435,7 @@ def offline_reports(args) -> int:
             pass
         t["kernel"] = args.kernel
         t["algorithmic_bytes"] = args.alg_bytes or 20 * args.n ** 3
-        t["n"] = args.n
+        t["n"] = args.grid_n or args.n
         # the schedule the profile pass ran (its own JSON line in the fetch pass's log)
         for log in sorted(Path(args.pmc_traffic[0]).parent.glob(Path(args.pmc_traffic[0]).name + "*.log")):
             for ln_ in log.read_text().splitlines():
@@ -460,6 +476,9 @@ def main() -> int:
     ap.add_argument("--cmd", type=str, default="", help="command line quoted in --stats-md output")
     ap.add_argument("--kernel", type=str, default="fused_kernel",
                     help="with --pmc-traffic: the kernel whose bytes per launch are reported")
+    ap.add_argument("--grid-n", type=int, default=0,
+                    help="with --pmc-traffic: edge of the profiled workload's grid (default --n, "
+                         "which sizes the calibration copy)")
     ap.add_argument("--alg-bytes", type=int, default=0,
                     help="with --pmc-traffic: algorithmic bytes per launch (default 20 n^3)")
     ap.add_argument("--tail", type=int, default=0,
