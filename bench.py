@@ -23,17 +23,12 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-STENCIL_GRAPH = "divergence(gradient(f)) + dot(u, gradient(f)) + sqnorm(curl(u))"
-REDUCTIONS = ("sum", "mean", "variance", "norm_l2")
 HBM_PEAK_GBS = 8000.0
 
 
 def build_graph(f, u):
-    import stenpy_amd as sp
-    g, sink, _, _, _ = sp.compile_expression(STENCIL_GRAPH, dx=1.0, boundary="periodic",
-                                             field_map={"f": f, "u": u})
-    reds = [g.add(r, (sink,), {}) for r in REDUCTIONS]
-    return g, sink, reds
+    from stenpy_amd.workloads import stencil_graph
+    return stencil_graph(f, u)
 
 
 def time_steps(fn, steps: int, warmup: int):
@@ -158,40 +153,13 @@ def _line(metric, value, unit, steps, warmup, ms, config, **extra):
 
 
 def c3_items(dev):
-    """The batched workload: 96 fields with axes U{32..384} (seed 6) and 6 random graphs of
-    3-12 ops (seed 6); field i (seed 1000 + i) uses graph i mod 6.  Returns (exprs, items)."""
-    import random
-    import stenpy_amd as sp
-    from stenpy_amd import gen, randgraph
-    rng = random.Random(6)
-    exprs = [randgraph.random_expression(rng, rng.randint(3, 12), vector_leaf=False, mid_mean=0.15)[0]
-             for _ in range(6)]
-    shapes = [tuple(rng.randint(32, 384) for _ in range(3)) for _ in range(96)]
-    items = []
-    for i, sh in enumerate(shapes):
-        f = gen.generate(sh, 1000 + i, dev)
-        g, sink, *_ = sp.compile_expression(exprs[i % 6], boundary="periodic", field_map={"f": f})
-        items.append((g, [sink]))
-    return exprs, items
-
-
-TENSOR_EXPRS = ["eigenvalues(hessian(f))", "determinant(hessian(f))", "trace(hessian(f))",
-                "deviatoric(hessian(f))", "mean_curvature(f)"]
+    from stenpy_amd.workloads import batched_items
+    return batched_items(dev)
 
 
 def c5_graph(dev, n: int = 768):
-    """The tensor workload: n^3 FP32 field, lower half the seeded Fourier field (seed 8), upper
-    half i^2 + j^2 + k^2; one graph whose five sinks share the hessian.  Returns (g, sinks, f)."""
-    import torch
-    from stenpy_amd import gen
-    from stenpy_amd.compiler import compile_expressions
-    f = gen.generate((n, n, n), 8, dev)
-    i = torch.arange(n, device=dev, dtype=torch.float32)
-    q = (i.view(-1, 1, 1) ** 2 + i.view(1, -1, 1) ** 2 + i.view(1, 1, -1) ** 2)
-    f[n // 2:] = q[n // 2:]
-    del q
-    g, sinks, *_ = compile_expressions(TENSOR_EXPRS, boundary="periodic", field_map={"f": f})
-    return g, sinks, f
+    from stenpy_amd.workloads import tensor_graph
+    return tensor_graph(dev, n)
 
 
 def bench_batched(args) -> int:
@@ -240,7 +208,7 @@ def bench_tensor(args) -> int:
     from stenpy_amd import gen, tune
     dev = torch.device("cuda:0")
     n = 768 if args.n == 512 else args.n
-    exprs = TENSOR_EXPRS
+    from stenpy_amd.workloads import TENSOR_EXPRS as exprs
     g, sinks, f = c5_graph(dev, n)
     graphs = [(g, sinks)]
     rt = sp.Runtime(device=dev, fusion_enabled=True, schedule_fn=tune.schedule_fn(dev))

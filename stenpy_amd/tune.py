@@ -122,19 +122,21 @@ def load_default_model(device="cuda") -> Optional["ScheduleModel"]:
     return _MODEL
 
 
-def schedule_fn(device: Optional[torch.device] = None, use_model: bool = True):
+def schedule_fn(device: Optional[torch.device] = None, use_model: bool = True,
+                threads: Optional[int] = None):
     """Schedule source for the fused executor: tuning database, then the learned model, then
-    (by returning None) the executor's shape heuristic."""
+    (by returning None) the executor's shape heuristic.  ``threads`` restricts both to one
+    block size (the batched launches share one)."""
     db = load_db()
 
     def pick(group, graph) -> Optional[Schedule]:
         hit = db.get(group_signature(group, graph))
-        if hit:
+        if hit and (threads is None or hit[3] == threads):
             return Schedule(*hit)
         if use_model:
             m = load_default_model(device or "cuda")
             if m is not None:
-                return m.best(group, graph)
+                return m.best(group, graph, threads)
         return None
 
     return pick
@@ -310,8 +312,11 @@ def default_time(rows: List[Dict[str, Any]], sched: Optional[Schedule] = None) -
 # ------------------------------------------------------------------------------ learned model
 def random_group_rows(n_graphs: int, seed: int, runtime, cands: Sequence[Schedule] = None,
                       shape_range: Tuple[int, int] = (64, 256), reps: int = 5, workers: int = 12,
-                      log=None) -> List[Dict[str, Any]]:
-    """Exhaustive timings of random op graphs (3-12 ops) on random grids (training data)."""
+                      log=None, slab_frac: float = 0.0) -> List[Dict[str, Any]]:
+    """Exhaustive timings of random op graphs (3-12 ops) on random grids (training data).
+
+    ``slab_frac``: share of the grids drawn slab-like instead (128-320 planes of a 1024-2048
+    square-ish cross-section), the shape of an out-of-core slab."""
     import random
     from . import randgraph
     from .compiler import compile_expression
@@ -321,6 +326,8 @@ def random_group_rows(n_graphs: int, seed: int, runtime, cands: Sequence[Schedul
     for gi in range(n_graphs):
         expr, nops = randgraph.random_expression(rng, rng.randint(3, 12))
         shape = randgraph.random_shape(rng, *shape_range)
+        if slab_frac > 0 and rng.random() < slab_frac:
+            shape = (rng.randint(128, 320), rng.randint(1024, 2048), rng.randint(1024, 2048))
         f = gen.generate(shape, 1000 + gi, runtime.device)
         u = gen.generate_vector(shape, (2000 + gi, 3000 + gi, 4000 + gi), runtime.device)
         g, sink, *_ = compile_expression(expr, boundary=rng.choice(["periodic", "neumann"]),
@@ -370,11 +377,11 @@ class ScheduleModel:
         X = np.asarray([r["features"] for r in rows], dtype=np.float32)
         return self.mlp.predict(X)
 
-    def best(self, group, graph) -> Optional[Schedule]:
-        s = self.shortlist(group, graph, 1)
+    def best(self, group, graph, threads: Optional[int] = None) -> Optional[Schedule]:
+        s = self.shortlist(group, graph, 1, threads)
         return s[0] if s else None
 
-    def shortlist(self, group, graph, k: int = 3) -> List[Schedule]:
+    def shortlist(self, group, graph, k: int = 3, threads: Optional[int] = None) -> List[Schedule]:
         info = getattr(group, "_info", None)
         try:
             base = F.KernelGen(graph, group, info, F.DEFAULT_SCHEDULE, {}, {})
@@ -383,6 +390,8 @@ class ScheduleModel:
         stats = group_stats(base)
         feats, scheds = [], []
         for s in self.cands:
+            if threads is not None and s.threads != threads:
+                continue
             try:
                 gen = F.KernelGen(graph, group, info, s, {}, {})
             except ValueError:
@@ -445,6 +454,110 @@ def evaluate_choice(rows: List[Dict[str, Any]], model: ScheduleModel, k: int = 1
             "geomean_ceiling_gain_vs_default": geo(ceiling)}
 
 
+def config_choice(rows: List[Dict[str, Any]], model: "ScheduleModel",
+                  db_sched: Optional[Sequence[int]] = None) -> Dict[str, Any]:
+    """One group's exhaustive timings against the model's picks.
+
+    The model ranks only the schedules it was trained on (``MODEL_CANDIDATES``); its top-1
+    pick and the best of its top 3 (as the shortlist search would time them) are compared
+    with the exhaustive best over *every* timed candidate, and with the best inside the
+    model's own candidate set."""
+    own = {c.as_tuple() for c in MODEL_CANDIDATES}
+    mine = [r for r in rows if tuple(r["schedule"]) in own]
+    best = min(rows, key=lambda r: r["ms"])
+    best_own = min(mine, key=lambda r: r["ms"])
+    order = np.argsort(np.asarray(model.predict_rows(mine)).reshape(-1))
+    top1 = mine[int(order[0])]
+    top3 = min((mine[int(i)] for i in order[:3]), key=lambda r: r["ms"])
+    out = {"grid": rows[0]["grid"], "timed": len(rows), "model_candidates": len(mine),
+           "best": {"schedule": best["schedule"], "ms": best["ms"]},
+           "best_in_model_set": {"schedule": best_own["schedule"], "ms": best_own["ms"]},
+           "model_top1": {"schedule": top1["schedule"], "ms": top1["ms"],
+                          "regret": top1["ms"] / best["ms"] - 1.0,
+                          "regret_in_model_set": top1["ms"] / best_own["ms"] - 1.0},
+           "model_top3": {"schedule": top3["schedule"], "ms": top3["ms"],
+                          "regret": top3["ms"] / best["ms"] - 1.0,
+                          "regret_in_model_set": top3["ms"] / best_own["ms"] - 1.0}}
+    h = default_time(rows)
+    if h is not None:
+        out["heuristic"] = {"schedule": list(F.heuristic_schedule(rows[0]["grid"]).as_tuple()), "ms": h,
+                            "regret": h / best["ms"] - 1.0}
+    if db_sched is not None:
+        hit = [r for r in rows if list(r["schedule"]) == list(db_sched)]
+        if hit:
+            out["database"] = {"schedule": list(db_sched), "ms": hit[0]["ms"],
+                               "regret": hit[0]["ms"] / best["ms"] - 1.0}
+    return out
+
+
+def config_groups(dev, which: Sequence[int] = (1, 2, 3, 4, 5)):
+    """(label, graph, wanted) of the benchmark configurations' fused evaluations; config 3
+    contributes the first field of each of its six graphs, config 4 one padded slab."""
+    from . import workloads as W
+    if 1 in which:
+        for e, g, w in W.plain_graphs(dev):
+            yield f"C1 128^3 {e}", g, w
+    if 2 in which:
+        f, u = W.stencil_inputs(dev)
+        g, sink, reds = W.stencil_graph(f, u)
+        del f, u
+        yield "C2 512^3 stencil graph", g, [sink] + reds
+    if 3 in which:
+        exprs, items = W.batched_items(dev)
+        for k in range(6):
+            g, w = items[k]
+            yield f"C3 graph {k} on field {k}: {exprs[k]}", g, w
+        del items
+    if 4 in which:
+        g, w = W.stream_slab_graph(dev)
+        yield "C4 slab (258, 4096, 4096)", g, w
+    if 5 in which:
+        g, sinks, _ = W.tensor_graph(dev)
+        yield "C5 768^3 tensor graph", g, sinks
+
+
+def config_regret(dev, which: Sequence[int] = (1, 2, 3, 4, 5), reps: int = 10, workers: int = 12,
+                  log=None) -> Dict[str, Any]:
+    """Exhaustive search over every candidate for each fused group of the benchmark
+    configurations, scored against the committed model's top-1 / top-3 picks."""
+    from .runtime import Runtime
+    model = load_default_model(dev)
+    if model is None:
+        raise RuntimeError(f"no trained schedule model at {MODEL_PATH}")
+    db = load_db()
+    report: Dict[str, Any] = {"groups": []}
+    for label, g, wanted in config_groups(dev, which):
+        if log:
+            log(f"== {label}")
+        rt = Runtime(device=dev, fusion_enabled=True)
+        rows = autotune(g, rt, wanted, cands=CANDIDATES, reps=reps, workers=workers, update_db=False,
+                        log=log)
+        by: Dict[str, List[Dict[str, Any]]] = {}
+        for r in rows:
+            by.setdefault(r["sig"], []).append(r)
+        for sig, rs in by.items():
+            entry = {"config": label, **config_choice(rs, model, db.get(sig))}
+            report["groups"].append(entry)
+            if log:
+                log(json.dumps({k: entry[k] for k in ("config", "best", "model_top1", "model_top3")}))
+        del rt, g
+        torch.cuda.empty_cache()
+    gs = report["groups"]
+    geo = lambda v: float(np.exp(np.mean(np.log(v))))
+    report["summary"] = {
+        "groups": len(gs),
+        "top1_exact": sum(e["model_top1"]["schedule"] == e["best"]["schedule"] for e in gs),
+        "top1_mean_regret": float(np.mean([e["model_top1"]["regret"] for e in gs])),
+        "top1_max_regret": float(np.max([e["model_top1"]["regret"] for e in gs])),
+        "top3_mean_regret": float(np.mean([e["model_top3"]["regret"] for e in gs])),
+        "top3_max_regret": float(np.max([e["model_top3"]["regret"] for e in gs])),
+        "top1_geomean_gain_vs_heuristic": geo([e["heuristic"]["ms"] / e["model_top1"]["ms"]
+                                               for e in gs if "heuristic" in e]),
+        "exhaustive_geomean_gain_vs_heuristic": geo([e["heuristic"]["ms"] / e["best"]["ms"]
+                                                     for e in gs if "heuristic" in e])}
+    return report
+
+
 # ------------------------------------------------------------------------------ command line
 def _read_rows(path) -> List[Dict[str, Any]]:
     import gzip
@@ -498,6 +611,9 @@ def main(argv=None) -> int:
     c.add_argument("--max-shape", type=int, default=256)
     c.add_argument("--time-limit", type=float, default=1e9, help="stop after this many seconds")
     c.add_argument("--workers", type=int, default=12)
+    c.add_argument("--slab-frac", type=float, default=0.0, help="share of slab-shaped grids")
+    c.add_argument("--all-candidates", action="store_true",
+                   help="time every candidate (512-plane z-chunks included), not the data subset")
     t = sub.add_parser("train", help="train the schedule MLP on collected rows")
     t.add_argument("--rows", default=str(ROWS_PATH))
     t.add_argument("--out", default=str(MODEL_PATH))
@@ -509,6 +625,11 @@ def main(argv=None) -> int:
     u.add_argument("--shape", required=True, help="N0xN1xN2")
     u.add_argument("--boundary", default="periodic")
     u.add_argument("--reductions", default="")
+    r = sub.add_parser("configs", help="exhaustive search on the benchmark configurations' groups, "
+                                       "scored against the trained model (regret report)")
+    r.add_argument("--which", default="1,2,3,4,5", help="comma-separated configuration numbers")
+    r.add_argument("--out", default=None, help="write the JSON report here")
+    r.add_argument("--workers", type=int, default=12)
     a = ap.parse_args(argv)
 
     from .runtime import Runtime
@@ -523,15 +644,27 @@ def main(argv=None) -> int:
             if time.time() - t0 > a.time_limit:
                 print(f"time limit reached after graph {gi - 1}", flush=True)
                 break
-            rows = random_group_rows(1, a.seed * 100003 + gi, rt, cands=DATA_CANDIDATES,
+            rows = random_group_rows(1, a.seed * 100003 + gi, rt,
+                                     cands=CANDIDATES if a.all_candidates else DATA_CANDIDATES,
                                      shape_range=(a.min_shape, a.max_shape), workers=a.workers,
-                                     log=lambda m: print(m, flush=True))
+                                     log=lambda m: print(m, flush=True), slab_frac=a.slab_frac)
             for r in rows:
                 r["graph"] = gi
             _append_rows(a.out, rows)
             n_rows += len(rows)
             rt.fused._plans.clear()
             print(f"[{time.time() - t0:.0f}s] graph {gi}: {len(rows)} rows, total {n_rows}", flush=True)
+        return 0
+    if a.cmd == "configs":
+        which = [int(v) for v in a.which.split(",") if v]
+        t0 = time.time()
+        rep = config_regret(dev, which, workers=a.workers,
+                            log=lambda m: print(f"[{time.time() - t0:.0f}s] {m}", flush=True))
+        rep["command"] = "python -m stenpy_amd.tune configs --which " + a.which
+        text = json.dumps(rep, indent=1) + "\n"
+        if a.out:
+            Path(a.out).write_text(text)
+        print(json.dumps(rep["summary"]), flush=True)
         return 0
     if a.cmd == "train":
         rows = _read_rows(a.rows)
