@@ -173,7 +173,12 @@ def bench_batched(args) -> int:
     dev = torch.device("cuda:0")
     exprs, items = c3_items(dev)
     t0 = time.perf_counter()
-    br = B.BatchRunner(items, device=dev)
+    # default: the batch's own shape heuristic.  The model's picks, ranked on single-field
+    # timings, favour short z-chunks that cost more inside one shared persistent launch (measured
+    # 217 against 247 Gvoxel/s); ``--schedules model`` runs them (one block size per module).
+    use_model = args.schedules == "model"
+    sfn = tune.schedule_fn(dev, threads=B.BATCH_THREADS) if use_model else None
+    br = B.BatchRunner(items, device=dev, schedule_fn=sfn)
     br.compile(workers=8)
     t_plan = time.perf_counter() - t0
     graph = br.capture()
@@ -192,6 +197,8 @@ def bench_batched(args) -> int:
           {"workload": "96 fields, axes U{32..384} seed 6; 6 random graphs (3-12 ops) seed 6; periodic",
            "exprs": exprs, "levels": st["levels"], "units": st["units"], "structures": st["structures"],
            "voxels": vox, "plan_compile_s": round(t_plan, 2),
+           "schedule_source": "tuning database, then the learned model" if use_model else "batch shape heuristic",
+           "schedules": br.schedules(),
            "sequential_fused_ms": round(t_seq * 1e3, 3), "speedup_vs_sequential": round(t_seq / t, 2)},
           roofline={"bound": "hbm", "achieved": round(st["algorithmic_bytes"] / t / 1e9, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -297,6 +304,7 @@ def bench_stream(args) -> int:
     streamed through pinned buffers; every slab by default (``--slabs N`` times the first N)."""
     import torch
     import stenpy_amd as sp
+    from stenpy_amd import tune
     from stenpy_amd.stream import GeneratorField, stream_graph
     from stenpy_amd.compiler import compile_expression
     dev = torch.device("cuda:0")
@@ -305,7 +313,8 @@ def bench_stream(args) -> int:
     probe = torch.zeros((1,) + shape[1:])
     g, sink, _, fm, _ = compile_expression("f**2 + tanh(f)*exp(f) + laplacian(f)", boundary="periodic",
                                            field_map={"f": probe})
-    rt = sp.Runtime(device=dev, fusion_enabled=True)
+    use_model = args.schedules != "heuristic"
+    rt = sp.Runtime(device=dev, fusion_enabled=True, schedule_fn=tune.schedule_fn(dev) if use_model else None)
     h2d = pinned_h2d_gbs(dev)
     st = stream_graph(g, sink, fm["f"], lazy, None, rt, boundary="periodic", slab_planes=256,
                       max_slabs=args.slabs or None)
@@ -316,6 +325,9 @@ def bench_stream(args) -> int:
                        f"{st['n_chunks']} of 18 slabs", "slabs": st["n_chunks"],
            "host_gen_s": round(st["host_gen_s"], 2), "l2": st["l2"], "mean": st["mean"],
            "min": st["min"], "max": st["max"], "captured_shapes": st["captured_shapes"],
+           "schedules": sorted({tuple(cg.gen.sched.as_tuple()) for plan in rt.fused._plans.values()
+                                for cg in plan[2].values()}),
+           "schedule_source": "tuning database, then the learned model" if use_model else "shape heuristic",
            "h2d_gb": round(st["h2d_bytes"] / 1e9, 2),
            "h2d_gbs_end_to_end": round(st["h2d_gbs_end_to_end"], 2),
            "pinned_h2d_gbs_probe": round(h2d, 2),
@@ -457,6 +469,9 @@ def main() -> int:
                          "random graphs in batched persistent launches; stream: out-of-core "
                          "4608x4096x4096 stream (bounded sample of slabs); tensor: 768^3 hessian "
                          "tensor ops; spectral: 512^3 spectral Laplacian")
+    ap.add_argument("--schedules", choices=["auto", "heuristic", "model"], default="auto",
+                    help="batched / stream schedule source; auto: the batch heuristic for batched, "
+                         "the tuning database then the learned model for stream")
     ap.add_argument("--slabs", type=int, default=0, help="stream: slabs of 256 planes to stream (0: all 18)")
     args = ap.parse_args()
     if args.pmc_traffic or args.stats_md:

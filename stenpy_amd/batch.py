@@ -231,6 +231,10 @@ class BatchRunner:
                   "}"]
         return "\n".join(lines)
 
+    def schedules(self) -> List[List[List[int]]]:
+        """The schedule of each kernel structure, per level (TX, TY, ZC, threads)."""
+        return [[list(g.sched.as_tuple()) for g in m["gens"]] for m in self.mods]
+
     # -- buffers and parameters ------------------------------------------------------------------
     def _allocate(self) -> None:
         dev = self.device

@@ -163,8 +163,8 @@ class ScheduleMLP:
                 tot += loss
                 nb += 1
             losses.append(float(tot.item()) / nb)
-            if verbose and ep % 20 == 0:
-                print(f"epoch {ep} loss {losses[-1]:.4f}")
+            if verbose and ep % 5 == 0:
+                print(f"epoch {ep} loss {losses[-1]:.4f}", flush=True)
         return losses
 
     def predict(self, X: np.ndarray) -> np.ndarray:

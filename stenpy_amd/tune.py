@@ -360,7 +360,7 @@ class ScheduleModel:
         self.cands = list(cands or MODEL_CANDIDATES)
 
     def fit(self, rows: List[Dict[str, Any]], device="cuda", epochs: int = 40, seed: int = 0,
-            weight_decay: float = 1e-3):
+            weight_decay: float = 1e-3, verbose: bool = False):
         from .mlp import ScheduleMLP
         X = np.asarray([r["features"] for r in rows], dtype=np.float32)
         y = np.log(np.asarray([r["ns_per_voxel"] for r in rows], dtype=np.float64))
@@ -371,7 +371,7 @@ class ScheduleModel:
             y[idx] -= y[idx].mean()
         self.mlp = ScheduleMLP(X.shape[1], 256, 4, device, seed)
         return self.mlp.fit(X, y.astype(np.float32), epochs=epochs, batch=256, lr=2e-3, seed=seed,
-                            weight_decay=weight_decay)
+                            weight_decay=weight_decay, verbose=verbose)
 
     def predict_rows(self, rows: List[Dict[str, Any]]) -> np.ndarray:
         X = np.asarray([r["features"] for r in rows], dtype=np.float32)
@@ -482,11 +482,38 @@ def config_choice(rows: List[Dict[str, Any]], model: "ScheduleModel",
     if h is not None:
         out["heuristic"] = {"schedule": list(F.heuristic_schedule(rows[0]["grid"]).as_tuple()), "ms": h,
                             "regret": h / best["ms"] - 1.0}
+    add_shortlist(out)
     if db_sched is not None:
         hit = [r for r in rows if list(r["schedule"]) == list(db_sched)]
         if hit:
             out["database"] = {"schedule": list(db_sched), "ms": hit[0]["ms"],
                                "regret": hit[0]["ms"] / best["ms"] - 1.0}
+    return out
+
+
+def add_shortlist(entry: Dict[str, Any]) -> Dict[str, Any]:
+    """The autotuner's shortlist search: the model's top 3 plus the shape heuristic, all
+    timed, the fastest kept (``entry`` as ``config_choice`` makes it)."""
+    picks = [entry["model_top3"]] + ([entry["heuristic"]] if "heuristic" in entry else [])
+    p = min(picks, key=lambda r: r["ms"])
+    entry["shortlist_top3_and_heuristic"] = {"schedule": p["schedule"], "ms": p["ms"],
+                                             "regret": p["ms"] / entry["best"]["ms"] - 1.0}
+    return entry
+
+
+def summarize_choices(groups: List[Dict[str, Any]]) -> Dict[str, Any]:
+    """Mean / max regret of each pick over the groups of a schedule study."""
+    geo = lambda v: float(np.exp(np.mean(np.log(v)))) if v else None
+    out: Dict[str, Any] = {"groups": len(groups),
+                           "top1_exact": sum(e["model_top1"]["schedule"] == e["best"]["schedule"] for e in groups)}
+    for key in ("model_top1", "model_top3", "shortlist_top3_and_heuristic", "heuristic", "database"):
+        vals = [e[key]["regret"] for e in groups if key in e]
+        if vals:
+            out[key] = {"groups": len(vals), "mean_regret": float(np.mean(vals)), "max_regret": float(np.max(vals))}
+    out["top1_geomean_gain_vs_heuristic"] = geo([e["heuristic"]["ms"] / e["model_top1"]["ms"]
+                                                 for e in groups if "heuristic" in e])
+    out["exhaustive_geomean_gain_vs_heuristic"] = geo([e["heuristic"]["ms"] / e["best"]["ms"]
+                                                       for e in groups if "heuristic" in e])
     return out
 
 
@@ -542,19 +569,7 @@ def config_regret(dev, which: Sequence[int] = (1, 2, 3, 4, 5), reps: int = 10, w
                 log(json.dumps({k: entry[k] for k in ("config", "best", "model_top1", "model_top3")}))
         del rt, g
         torch.cuda.empty_cache()
-    gs = report["groups"]
-    geo = lambda v: float(np.exp(np.mean(np.log(v))))
-    report["summary"] = {
-        "groups": len(gs),
-        "top1_exact": sum(e["model_top1"]["schedule"] == e["best"]["schedule"] for e in gs),
-        "top1_mean_regret": float(np.mean([e["model_top1"]["regret"] for e in gs])),
-        "top1_max_regret": float(np.max([e["model_top1"]["regret"] for e in gs])),
-        "top3_mean_regret": float(np.mean([e["model_top3"]["regret"] for e in gs])),
-        "top3_max_regret": float(np.max([e["model_top3"]["regret"] for e in gs])),
-        "top1_geomean_gain_vs_heuristic": geo([e["heuristic"]["ms"] / e["model_top1"]["ms"]
-                                               for e in gs if "heuristic" in e]),
-        "exhaustive_geomean_gain_vs_heuristic": geo([e["heuristic"]["ms"] / e["best"]["ms"]
-                                                     for e in gs if "heuristic" in e])}
+    report["summary"] = summarize_choices(report["groups"])
     return report
 
 
@@ -583,12 +598,12 @@ def split_by_graph(rows: List[Dict[str, Any]], holdout_every: int = 5):
 
 
 def train_and_report(rows: List[Dict[str, Any]], device="cuda", epochs: int = 40,
-                     seed: int = 0) -> Tuple["ScheduleModel", Dict[str, Any]]:
+                     seed: int = 0, verbose: bool = False) -> Tuple["ScheduleModel", Dict[str, Any]]:
     """Fit on 4/5 of the graphs, report regret / gain on the held-out fifth (pick = the top
     prediction, and the best of the top three as the shortlist search would time them)."""
     train, test = split_by_graph(rows)
     model = ScheduleModel(cands=MODEL_CANDIDATES)
-    losses = model.fit(train, device=device, epochs=epochs, seed=seed)
+    losses = model.fit(train, device=device, epochs=epochs, seed=seed, verbose=verbose)
     rep = {"rows_train": len(train), "rows_test": len(test), "epochs": epochs,
            "graphs": len({r.get("graph") for r in rows}),
            "loss_first": losses[0], "loss_last": losses[-1],
@@ -668,7 +683,7 @@ def main(argv=None) -> int:
         return 0
     if a.cmd == "train":
         rows = _read_rows(a.rows)
-        model, rep = train_and_report(rows, device=dev, epochs=a.epochs)
+        model, rep = train_and_report(rows, device=dev, epochs=a.epochs, verbose=True)
         model.save(a.out)
         Path(a.report).write_text(json.dumps(rep, indent=1) + "\n")
         print(json.dumps(rep), flush=True)

@@ -12,6 +12,12 @@ class _RankModel:
         return np.asarray([r["score"] for r in rows], dtype=np.float32)
 
 
+def _restrict(monkeypatch):
+    """A model candidate set without the 512-plane z-chunks, so that some timed rows lie
+    outside it."""
+    monkeypatch.setattr(tune, "MODEL_CANDIDATES", [c for c in tune.CANDIDATES if c.zc <= 256])
+
+
 def _rows():
     own = tune.MODEL_CANDIDATES[:5]
     wide = [c for c in tune.CANDIDATES if c.zc == 512][:2]     # outside the model's set
@@ -21,7 +27,8 @@ def _rows():
             for c, m, s in zip(own + wide, ms, score)]
 
 
-def test_config_choice_scores_top1_and_top3_against_the_exhaustive_best():
+def test_config_choice_scores_top1_and_top3_against_the_exhaustive_best(monkeypatch):
+    _restrict(monkeypatch)
     rows = _rows()
     r = tune.config_choice(rows, _RankModel(), db_sched=rows[5]["schedule"])
     assert r["timed"] == 7 and r["model_candidates"] == 5
@@ -35,7 +42,8 @@ def test_config_choice_scores_top1_and_top3_against_the_exhaustive_best():
     assert r["database"]["regret"] == 0.0
 
 
-def test_config_choice_never_picks_outside_the_model_candidates():
+def test_config_choice_never_picks_outside_the_model_candidates(monkeypatch):
+    _restrict(monkeypatch)
     rows = _rows()
     r = tune.config_choice(rows, _RankModel())
     own = {tuple(c.as_tuple()) for c in tune.MODEL_CANDIDATES}
