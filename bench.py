@@ -54,14 +54,22 @@ def group_launch(rt, g, res, sink, out_dtype=None):
 
 
 def cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it, with family / model numbers (a virtual machine
+    may report only a generic name such as "AMD EPYC") and the visible CPU count."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("model name", "cpu family", "model", "vendor_id") and k not in info:
+                info[k] = v.strip()
     except OSError:
         pass
     import platform
-    return platform.processor() or "unknown"
+    name = info.get("model name") or platform.processor() or "unknown"
+    if "cpu family" in info:
+        name += f" (family {info['cpu family']}, model {info.get('model', '?')})"
+    return f"{name}, {os.cpu_count()} CPUs visible"
 
 
 def cpu_threads() -> int:

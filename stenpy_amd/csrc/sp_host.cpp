@@ -25,33 +25,35 @@ inline uint64_t splitmix_at(uint64_t seed, uint64_t lin) {
 }
 
 // The row loop vectorises over k once 64-bit multiplies and u64 -> double conversions are
-// vector instructions (AVX-512 DQ); AVX2 and plain builds keep the scalar noise term.
+// vector instructions (AVX-512 DQ); AVX2 and plain builds keep the scalar noise term.  A call
+// covers columns [0, len) of one row segment; the tables C, S are offset to the segment's first
+// column and keep their row stride ts (one row of 8 modes per table).
 #define SP_ROW_BODY                                                                                  \
-    for (int64_t k = 0; k < n2; ++k) {                                                               \
+    for (int64_t k = 0; k < len; ++k) {                                                              \
         double acc = 0.0;                                                                            \
-        for (int m = 0; m < 8; ++m) acc += cr[m] * C[m * n2 + k] - ci[m] * S[m * n2 + k];           \
+        for (int m = 0; m < 8; ++m) acc += cr[m] * C[m * ts + k] - ci[m] * S[m * ts + k];           \
         const double u = (double)(splitmix_at(seed, lin0 + (uint64_t)k) >> 11) * (1.0 / 9007199254740992.0); \
         out[k] = (float)(acc + (2.0 * u - 1.0) * 1e-3);                                              \
     }
 
-using RowFn = void (*)(float*, int64_t, const double*, const double*, const double*, const double*,
-                       uint64_t, uint64_t);
+using RowFn = void (*)(float*, int64_t, int64_t, const double*, const double*, const double*,
+                       const double*, uint64_t, uint64_t);
 
 __attribute__((target("avx512f,avx512dq,avx512vl,avx512bw,fma")))
-void row_avx512dq(float* __restrict__ out, int64_t n2, const double* __restrict__ cr,
+void row_avx512dq(float* __restrict__ out, int64_t len, int64_t ts, const double* __restrict__ cr,
                   const double* __restrict__ ci, const double* __restrict__ C,
                   const double* __restrict__ S, uint64_t seed, uint64_t lin0) {
     SP_ROW_BODY
 }
 
 __attribute__((target("avx2,fma")))
-void row_avx2(float* __restrict__ out, int64_t n2, const double* __restrict__ cr,
+void row_avx2(float* __restrict__ out, int64_t len, int64_t ts, const double* __restrict__ cr,
               const double* __restrict__ ci, const double* __restrict__ C,
               const double* __restrict__ S, uint64_t seed, uint64_t lin0) {
     SP_ROW_BODY
 }
 
-void row_plain(float* __restrict__ out, int64_t n2, const double* __restrict__ cr,
+void row_plain(float* __restrict__ out, int64_t len, int64_t ts, const double* __restrict__ cr,
                const double* __restrict__ ci, const double* __restrict__ C,
                const double* __restrict__ S, uint64_t seed, uint64_t lin0) {
     SP_ROW_BODY
@@ -63,6 +65,12 @@ RowFn pick_row() {
     if (__builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma")) return row_avx2;
     return row_plain;
 }
+
+// Rows are taken in blocks of ROW_BLOCK and columns in segments of COL_SEG: the tables of one
+// segment (8 modes x 2 x COL_SEG doubles = 32 KiB) stay in the core's L1 while every row of the
+// block runs over it, instead of streaming both whole tables from L2 once per row.
+constexpr int64_t ROW_BLOCK = 16;
+constexpr int64_t COL_SEG = 256;
 
 }  // namespace
 
@@ -80,20 +88,30 @@ extern "C" int sp_host_gen_fourier(float* out, const int64_t n[3], int64_t z0, i
             S[m * n2 + k] = std::sin(t);
         }
     const int64_t rows = nz * n1;
+    const int64_t blocks = (rows + ROW_BLOCK - 1) / ROW_BLOCK;
     static const RowFn row = pick_row();
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel for schedule(static)
-    for (int64_t r = 0; r < rows; ++r) {
-        const int64_t i = z0 + r / n1, j = r % n1;
-        double cr[8], ci[8];
-        for (int m = 0; m < 8; ++m) {
-            const double t = two_pi * ((double)kk[3 * m] * (double)i / (double)n0 +
-                                       (double)kk[3 * m + 1] * (double)j / (double)n1) + phi[m];
-            cr[m] = a[m] * std::cos(t);
-            ci[m] = a[m] * std::sin(t);
+    for (int64_t b = 0; b < blocks; ++b) {
+        const int64_t r0 = b * ROW_BLOCK, r1 = r0 + ROW_BLOCK < rows ? r0 + ROW_BLOCK : rows;
+        double cr[ROW_BLOCK][8], ci[ROW_BLOCK][8];
+        for (int64_t r = r0; r < r1; ++r) {
+            const int64_t i = z0 + r / n1, j = r % n1;
+            for (int m = 0; m < 8; ++m) {
+                const double t = two_pi * ((double)kk[3 * m] * (double)i / (double)n0 +
+                                           (double)kk[3 * m + 1] * (double)j / (double)n1) + phi[m];
+                cr[r - r0][m] = a[m] * std::cos(t);
+                ci[r - r0][m] = a[m] * std::sin(t);
+            }
         }
-        row(out + r * n2, n2, cr, ci, C.data(), S.data(), seed,
-                   (uint64_t)((i * n1 + j) * n2));
+        for (int64_t k0 = 0; k0 < n2; k0 += COL_SEG) {
+            const int64_t len = k0 + COL_SEG < n2 ? COL_SEG : n2 - k0;
+            for (int64_t r = r0; r < r1; ++r) {
+                const int64_t i = z0 + r / n1, j = r % n1;
+                row(out + r * n2 + k0, len, n2, cr[r - r0], ci[r - r0], C.data() + k0, S.data() + k0,
+                    seed, (uint64_t)((i * n1 + j) * n2 + k0));
+            }
+        }
     }
     return 0;
 }

@@ -73,6 +73,16 @@ def test_generator_field_matches_host_formula():
     np.testing.assert_array_equal(out, full[26:30])
 
 
+def test_generator_field_rows_span_several_column_segments():
+    """Rows longer than the generator's 256-column segments and row counts that leave a
+    partial 16-row block: every voxel still equals the numpy formula."""
+    shape = (5, 23, 601)
+    lf = GeneratorField(shape, 11)
+    full = gen.generate_host(shape, 11)
+    np.testing.assert_array_equal(lf[1:4], full[1:4])
+    np.testing.assert_array_equal(lf[0:5], full)
+
+
 def test_npy_lazy_field(tmp_path):
     a = gen.generate_host((12, 8, 9), 3)
     p = tmp_path / "f.npy"
