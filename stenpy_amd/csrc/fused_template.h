@@ -172,19 +172,6 @@ __device__ __forceinline__ float quad_weight(int i, int n, float dx, int method)
     return w;
 }
 
-// 1 / max(sqrt(s), eps) in FP64 for a sum of squares s >= 0 (the unit normal of a gradient):
-// the FP64 reciprocal square root refined by one Newton step, relative error about 2^-45, so
-// x times it rounded to FP32 is x / max(sqrt(s), eps) rounded once (the FP32 formula rounds the
-// square root and the quotient).  NaN s gives NaN and an infinite s gives 0, as that formula.
-__device__ __forceinline__ double inv_norm64(float s, float eps) {
-    const double d = (double)s, e = (double)eps;
-    const double r0 = __builtin_amdgcn_rsq(d);
-    const double r = r0 * __builtin_fma(-0.5 * d, r0 * r0, 1.5);
-    // selects, not branches: clamped (or NaN) below eps^2, 0 for an infinite s
-    const double lo = d != d ? d : 1.0 / e;
-    return d >= e * e ? (d > 3.5e38 ? 0.0 : r) : lo;
-}
-
 // torch.gradient one-sided formulas (interior ((x+)-(x-))/h/2)
 __device__ __forceinline__ float tg_mid(float xp, float xm, float h) { return ((xp - xm) / h) * 0.5f; }
 __device__ __forceinline__ float tg_left2(float x0, float x1, float x2, float h) {

@@ -53,10 +53,26 @@ __device__ __forceinline__ float sym3_det_regularised(const float* m, float reg)
 #if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC_RTC__)
 #define sym3_sqrt(x) __builtin_amdgcn_sqrtf(x)
 #define sym3_rcp(x) __builtin_amdgcn_rcpf(x)
+#define sym3_rsq(x) __builtin_amdgcn_rsqf(x)
 #else
 #define sym3_sqrt(x) sqrtf(x)
 #define sym3_rcp(x) (1.0f / (x))
+#define sym3_rsq(x) (1.0f / sqrtf(x))
 #endif
+
+// 1 / max(sqrt(s), eps) for a sum of squares s >= 0, the scale of a unit normal g / max(|g|, eps):
+// the hardware reciprocal square root (1 ulp) refined by one Newton step in FP32,
+// r = r0 + r0 (1/2 - (s r0) r0 / 2), with the product s r0 rounded and the rest fused.  Each
+// component g_i r is then within about 2 ulp of g_i / max(|g|, eps), against about 1.5 ulp for the
+// FP32 formula that rounds the norm and the quotient.  Selects, not branches: 1 / eps below eps^2,
+// NaN for NaN, 0 for an infinite s (as the formula).
+__device__ __forceinline__ float unit_rnorm(float s, float eps) {
+    const float r0 = sym3_rsq(s);
+    const float e = fmaf(-0.5f * (s * r0), r0, 0.5f);
+    const float r = fmaf(r0, e, r0);
+    const float lo = s != s ? s : 1.0f / eps;
+    return s >= eps * eps ? (s > 3.4028234663852886e38f ? 0.0f : r) : lo;
+}
 
 // atan2(y, x) for y >= 0 (or NaN), the closed form's angle in [0, pi]: a = min / max of |x| and
 // y (the hardware reciprocal and one FMA correction), atan a = a + a^3 P(a^2) on [0, 1] with P of

@@ -811,12 +811,11 @@ class _Emit:
         if kind == "nrm":               # unit normal of gradient_nd
             _, inner, dxs, o, eps = vid
             gs = [self.d1(inner, 0, pos, a, dxs[a], o) for a in range(3)]
-            # g / max(|g|, eps) with one rounding: the FP64 reciprocal norm of the FP32 sum of
-            # squares (a reciprocal square root and one Newton step) instead of a correctly
+            # g / max(|g|, eps): the FP32 reciprocal norm of the FP32 sum of squares (hardware
+            # reciprocal square root and one Newton step) times each component, instead of a
             # rounded square root and a division per component
-            r = self.var(f"inv_norm64(({gs[0]} * {gs[0]} + {gs[1]} * {gs[1]}) + {gs[2]} * {gs[2]}, {_flt(eps)})",
-                         "double")
-            return [self.var(f"(float)((double){x} * {r})") for x in gs]
+            r = self.var(f"unit_rnorm(({gs[0]} * {gs[0]} + {gs[1]} * {gs[1]}) + {gs[2]} * {gs[2]}, {_flt(eps)})")
+            return [self.var(f"{x} * {r}") for x in gs]
         raise KeyError(kind)
 
     def _tensor(self, op: str, m: List[str]) -> List[str]:

@@ -23,6 +23,7 @@ extern "C" void div3(const float* x, float* y, long n) { for (long i = 0; i < n;
 extern "C" void eig(const float* m, float* e, long n) { for (long i = 0; i < n; ++i) sym3_eigvalsh(m + 9 * i, e + 3 * i); }
 extern "C" void dev(const float* m, float* d, long n) { for (long i = 0; i < n; ++i) sym3_deviatoric(m + 9 * i, d + 9 * i); }
 extern "C" void atan2u(const float* y, const float* x, float* r, long n) { for (long i = 0; i < n; ++i) r[i] = atan2_upper(y[i], x[i]); }
+extern "C" void rnorm(const float* s, float eps, float* r, long n) { for (long i = 0; i < n; ++i) r[i] = unit_rnorm(s[i], eps); }
 """
 
 
@@ -39,6 +40,7 @@ def sym3(tmp_path_factory):
     for fn in (lib.div3, lib.eig, lib.dev):
         fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
     lib.atan2u.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
+    lib.rnorm.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_long]
     return lib
 
 
@@ -112,3 +114,23 @@ def test_atan2_upper_against_numpy(sym3):
     out = np.empty(2, dtype=np.float32)
     sym3.atan2u(nan[0].ctypes.data, nan[1].ctypes.data, out.ctypes.data, 2)
     assert np.isnan(out).all()
+
+
+def test_unit_rnorm_values(sym3):
+    """1 / max(sqrt(s), eps) after one FP32 Newton step: within 1 ulp of the FP64 value over 40
+    binades of s (from a correctly rounded start here; the device starts from the 1-ulp hardware
+    reciprocal square root, which the step absorbs), and the clamp / NaN / inf cases."""
+    rng = np.random.default_rng(5)
+    s = np.exp(rng.uniform(np.log(1e-20), np.log(1e20), 200000)).astype(np.float32)
+    eps = np.float32(1e-15)
+    r = np.empty_like(s)
+    sym3.rnorm(s.ctypes.data, float(eps), r.ctypes.data, s.size)
+    exact = 1.0 / np.sqrt(s.astype(np.float64))
+    ulp = np.spacing(exact.astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(r - exact) / ulp) <= 1.0
+    special = np.array([0.0, 1e-31, np.inf, np.nan, 1e-30], dtype=np.float32)
+    out = np.empty_like(special)
+    sym3.rnorm(special.ctypes.data, float(eps), out.ctypes.data, special.size)
+    assert out[0] == np.float32(1.0) / eps and out[1] == np.float32(1.0) / eps
+    assert out[2] == 0.0 and np.isnan(out[3])
+    assert abs(out[4] - 1e15) <= 1e15 * 2e-7
