@@ -68,6 +68,25 @@ int sp_op_unary(int op, const sp_tensor* a, sp_tensor* out, double p0, double p1
 int sp_op_binary(int op, const sp_tensor* a, const sp_tensor* b, sp_tensor* out, double p0,
                  void* stream);
 
+/* Scalar programs: n <= SP_SCALAR_PROGRAM_MAX point-wise ops on single FP64 values, evaluated
+ * in order by one launch (an op may read an earlier op's result), in FP64.  Each op is a unary
+ * (binary = 0: SP_NEG .. SP_SQUARE, p0 / p1 as for sp_op_unary) or binary op (binary = 1:
+ * SP_ADD .. SP_DIV, DIV's eps in p0) on device doubles ``a`` / ``b``; a NULL pointer takes the
+ * constant av / bv instead.  The descriptors are copied into the launch's arguments, so the
+ * launch can be captured.  Replaces one launch per op of the reference's scalar arithmetic on
+ * reduction results (the point-wise ops of stenpy_engine.py:1218-1367 on 0-d tensors, run
+ * between the levels of a fused or batched graph). */
+typedef struct sp_scalar_op {
+    const double* a;
+    const double* b;
+    double* out;
+    double av, bv, p0, p1;
+    int32_t binary;
+    int32_t op;
+} sp_scalar_op;
+#define SP_SCALAR_PROGRAM_MAX 48
+int sp_scalar_program(const sp_scalar_op* ops, int n, void* stream);
+
 /* ---- channel-last helpers (stenpy_engine.py:2247-2313) --------------------------------- */
 /* out[..., :] = sum_c a[..., c] * b[..., c] folded left (extension op ``dot``). */
 int sp_op_dot(const sp_tensor* a, const sp_tensor* b, sp_tensor* out, void* stream);

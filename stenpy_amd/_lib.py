@@ -10,7 +10,7 @@ import ctypes as C
 import os
 import threading
 from pathlib import Path
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 import torch
@@ -97,6 +97,7 @@ def _declare(lib: C.CDLL) -> None:
         "sp_reduce_combine": ([I, V, I, C.c_int64, V, V], I),
         "sp_reduce_combine_batch": ([V, I, V], I),
         "sp_reduce_combine_multi": ([V, I, V], I),
+        "sp_scalar_program": ([V, I, V], I),
         "sp_tensor3": ([I, P, P, V, V], I),
         "sp_gen_fourier": ([P, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int32),
                             C.POINTER(D), C.POINTER(D), C.c_uint64, V], I),
@@ -192,6 +193,26 @@ def binary(op: str, a: torch.Tensor, b: torch.Tensor, out: torch.Tensor, eps: fl
     check(lib().sp_op_binary(BINARY[op], C.byref(desc(a)), C.byref(desc(b)), C.byref(desc(out)),
                              eps, stream_ptr(a.device)), f"binary {op}")
     return out
+
+
+class SpScalarOp(C.Structure):
+    """One op of ``sp_scalar_program`` (include/stenpy_hip.h)."""
+    _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("out", C.c_void_p),
+                ("av", C.c_double), ("bv", C.c_double), ("p0", C.c_double), ("p1", C.c_double),
+                ("binary", C.c_int32), ("op", C.c_int32)]
+
+
+SCALAR_PROGRAM_MAX = 48
+
+
+def scalar_program(ops: List[SpScalarOp], stream: Optional[int] = None, device=None) -> None:
+    """Evaluate the scalar ops in order, SCALAR_PROGRAM_MAX per launch (later chunks see the
+    earlier chunks' results, written in stream order)."""
+    st = stream if stream is not None else stream_ptr(device)
+    for i in range(0, len(ops), SCALAR_PROGRAM_MAX):
+        chunk = ops[i:i + SCALAR_PROGRAM_MAX]
+        arr = (SpScalarOp * len(chunk))(*chunk)
+        check(lib().sp_scalar_program(C.addressof(arr), len(chunk), st), "scalar program")
 
 
 def channel(kind: str, a: torch.Tensor, out: torch.Tensor, b: torch.Tensor = None,

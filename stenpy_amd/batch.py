@@ -381,12 +381,25 @@ class BatchRunner:
         st = stream if stream is not None else L.stream_ptr(self.device)
         from .registry import OP_REGISTRY
         for lvl, ln in enumerate(self.launches):
+            # the level's scalar steps (ops on earlier levels' reduction results) as one device
+            # program; a step it cannot express runs through the registry, in order
+            pending = []
             for ii, nid in self.pre_steps.get(lvl, []):
                 graph = self.items[ii][0]
                 node = graph.node(nid)
                 res = self.results[ii]
-                v = OP_REGISTRY[node.op_name](*[res[i] for i in node.input_ids], **node.params)
+                ins = [res[i] for i in node.input_ids]
+                d = F.scalar_op(node, ins, res[nid])
+                if d is not None:
+                    pending.append(d)
+                    continue
+                if pending:
+                    L.scalar_program(pending, st)
+                    pending = []
+                v = OP_REGISTRY[node.op_name](*ins, **node.params)
                 res[nid].copy_(torch.as_tensor(v, device=self.device).reshape(-1)[:1].to(torch.float64))
+            if pending:
+                L.scalar_program(pending, st)
             if ln["nunits"] == 0:
                 continue
             m = self.mods[lvl]

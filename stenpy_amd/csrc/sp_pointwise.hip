@@ -301,3 +301,73 @@ extern "C" int sp_op_normalize_last(const sp_tensor* a, sp_tensor* out, double e
                                     void* stream) {
     return channel_impl<3>(a, nullptr, out, (float)eps, stream, "sp_op_normalize_last");
 }
+
+// ---- scalar programs: the scalar steps between fused levels (ops on reduction results) ----
+namespace {
+
+struct ScalarProgram { sp_scalar_op op[SP_SCALAR_PROGRAM_MAX]; int n; };
+
+__device__ double scalar_unary(int op, double a, double p0, double p1) {
+    switch (op) {
+        case SP_NEG: return -a;
+        case SP_EXP: return exp(a);
+        case SP_LOG: return log(a < p0 ? p0 : a);
+        case SP_SQRT: return sqrt(a < p0 ? p0 : a);
+        case SP_SIN: return sin(a);
+        case SP_COS: return cos(a);
+        case SP_TANH: return tanh(a);
+        case SP_CLAMP: return a < p0 ? p0 : (a > p1 ? p1 : a);
+        case SP_SCALE: return a * p0;
+        case SP_SQUARE: return a * a;
+        default: return a;
+    }
+}
+
+__device__ double scalar_binary(int op, double a, double b, double eps) {
+    switch (op) {
+        case SP_ADD: return a + b;
+        case SP_SUB: return a - b;
+        case SP_MUL: return a * b;
+        default: return a / (fabs(b) < eps ? (b < 0.0 ? -eps : eps) : b);
+    }
+}
+
+// An operand that an earlier op of the program wrote is taken from that op's result as held
+// in registers, never re-read from memory (so no store-to-load ordering is needed).
+__device__ double scalar_operand(const ScalarProgram& prog, const double* vals, int i,
+                                 const double* ptr, double constant) {
+    if (!ptr) return constant;
+    for (int j = i - 1; j >= 0; --j)
+        if (prog.op[j].out == ptr) return vals[j];
+    return *ptr;
+}
+
+// One lane runs the ops in order, so an op may read the result of an earlier one.
+__global__ void k_scalar_program(const ScalarProgram prog) {
+    if (threadIdx.x != 0) return;
+    double vals[SP_SCALAR_PROGRAM_MAX];
+    for (int i = 0; i < prog.n; ++i) {
+        const sp_scalar_op& o = prog.op[i];
+        const double a = scalar_operand(prog, vals, i, o.a, o.av);
+        const double b = scalar_operand(prog, vals, i, o.b, o.bv);
+        vals[i] = o.binary ? scalar_binary(o.op, a, b, o.p0) : scalar_unary(o.op, a, o.p0, o.p1);
+        *o.out = vals[i];
+    }
+}
+
+}  // namespace
+
+extern "C" int sp_scalar_program(const sp_scalar_op* ops, int n, void* stream) {
+    SP_REQUIRE(ops && n > 0 && n <= SP_SCALAR_PROGRAM_MAX, "scalar_program: bad arguments");
+    ScalarProgram prog;
+    for (int i = 0; i < n; ++i) {
+        const sp_scalar_op& o = ops[i];
+        SP_REQUIRE(o.out, "scalar_program: missing output");
+        SP_REQUIRE(o.binary ? (o.op >= SP_ADD && o.op <= SP_DIV) : (o.op >= SP_NEG && o.op <= SP_SQUARE),
+                   "scalar_program: unknown op");
+        prog.op[i] = o;
+    }
+    prog.n = n;
+    hipLaunchKernelGGL(k_scalar_program, dim3(1), dim3(64), 0, as_stream(stream), prog);
+    return launch_status("sp_scalar_program");
+}

@@ -2004,6 +2004,62 @@ def pack_params(cg: CompiledGroup, srcs: Sequence[torch.Tensor], outs: Sequence[
     return struct.pack(f"<{len(ptrs)}Q6i", *ptrs, tx, ty, zcn, nb, int(zwin[0]), int(zwin[1]))
 
 
+# ------------------------------------------------------------------------------ scalar steps
+_SCALAR_UNARY = {"neg", "exp", "log", "sqrt", "sin", "cos", "tanh", "clamp"}
+_SCALAR_BINARY = {"add", "sub", "mul", "div"}
+
+
+def scalar_op(node: GraphNode, ins: Sequence[Any], out: torch.Tensor) -> Optional["L.SpScalarOp"]:
+    """The ``sp_scalar_program`` op computing ``node`` (a point-wise op whose operands are
+    single FP64 device values or Python numbers) into the FP64 device value ``out``; None when
+    the node is not such an op.  Parameters follow the registered ops' defaults
+    (stenpy_engine.py:1248-1367: log clamps at 1e-15, sqrt at 0, clamp to [0, 1], div
+    replaces |b| < 1e-15 by +-1e-15)."""
+    op, p = node.op_name, node.params
+    if op not in _SCALAR_UNARY and op not in _SCALAR_BINARY:
+        return None
+    if out.dtype != torch.float64 or out.numel() != 1 or not out.is_cuda:
+        return None
+    vals = []
+    for v in ins:
+        if isinstance(v, torch.Tensor) and not v.is_cuda and v.numel() == 1:
+            vals.append((None, float(v.reshape(-1)[0])))     # a seeded host scalar
+        elif isinstance(v, torch.Tensor):
+            if v.dtype != torch.float64 or v.numel() != 1 or v.device != out.device:
+                return None
+            vals.append((v.data_ptr(), 0.0))
+        elif isinstance(v, (int, float)) and not isinstance(v, bool):
+            vals.append((None, float(v)))
+        else:
+            return None
+    d = L.SpScalarOp()
+    d.out = out.data_ptr()
+    if op in _SCALAR_BINARY and not (op == "mul" and len(vals) == 1):
+        if len(vals) != 2:
+            return None
+        d.binary, d.op = 1, L.BINARY[op]
+        (d.a, d.av), (d.b, d.bv) = vals
+        d.p0 = float(p.get("eps", 1e-15)) if op == "div" else 0.0
+        return d
+    if len(vals) != 1:
+        return None
+    d.binary = 0
+    (d.a, d.av) = vals[0]
+    if op == "mul":                                  # mul(a, scalar=c)
+        if p.get("scalar") is None:
+            return None
+        d.op, d.p0 = L.UNARY["scale"], float(p["scalar"])
+    else:
+        d.op = L.UNARY[op]
+        if op == "log":
+            d.p0 = float(p.get("eps", 1e-15))
+        elif op == "sqrt":
+            d.p0 = float(p.get("eps", 0.0))
+        elif op == "clamp":
+            d.p0, d.p1 = float(p.get("lo", 0.0)), float(p.get("hi", 1.0))
+    return d
+
+
 # ------------------------------------------------------------------------------ executor
 class FusedExecutor:
     """Runs graphs through fused plans; caches plans by (graph structure, shapes, schedule)."""
@@ -2106,15 +2162,32 @@ class FusedExecutor:
         if wanted is None:
             wanted = [s for s in graph.sinks() if graph.node(s).op_name != "_constant"]
         steps, info, compiled = self.prepare(graph, values, wanted, out_dtype, schedule)
+        pending: List[Any] = []           # scalar steps gathered into one device program
+
+        def flush():
+            if pending:
+                L.scalar_program(pending, device=self.device)
+                pending.clear()
+
         for si, st in enumerate(steps):
             if st.kind == "eager":
                 node = graph.node(st.node)
                 ins = [results[i] for i in node.input_ids]
+                if info[st.node].ch == 0 and node.op_name not in REDUCTION_OPS:
+                    out = torch.empty((), dtype=torch.float64, device=self.device)
+                    d = scalar_op(node, ins, out)
+                    if d is not None:
+                        pending.append(d)
+                        results[st.node] = out
+                        continue
+                flush()
                 if red_window is not None and node.op_name in REDUCTION_OPS:
                     ins[0] = ins[0][red_window[0]:red_window[1]]
                 results[st.node] = OP_REGISTRY[node.op_name](*ins, **node.params)
                 continue
+            flush()
             self._run_group(compiled[si], graph, results, red_window)
+        flush()
         return results
 
     def capture(self, graph: Graph, runtime, wanted: Sequence[str],

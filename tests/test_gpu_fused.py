@@ -215,3 +215,44 @@ def test_reference_fused_entry_compile_and_execute():
     o64 = graph_eval.evaluate(g, torch.float64)[sink]
     o32 = graph_eval.evaluate(g, torch.float32)[sink]
     assert_2x(res[sink], o64, o32, "compile_and_execute")
+
+
+def test_scalar_program_ops_and_chaining():
+    """sp_scalar_program evaluates its ops in order in FP64: every unary and binary op against
+    Python's math, constants on either side, and an op reading an earlier op's result."""
+    import math
+    from stenpy_amd import _lib as L
+    dev = torch.device("cuda", 0)
+    x = torch.tensor([0.37], dtype=torch.float64, device=dev)
+    y = torch.tensor([-2.5], dtype=torch.float64, device=dev)
+    outs = [torch.empty((), dtype=torch.float64, device=dev) for _ in range(16)]
+    U, B = L.UNARY, L.BINARY
+
+    def op(i, binary, code, a=None, b=None, av=0.0, bv=0.0, p0=0.0, p1=0.0):
+        d = L.SpScalarOp()
+        d.a = a.data_ptr() if a is not None else None
+        d.b = b.data_ptr() if b is not None else None
+        d.out, d.av, d.bv, d.p0, d.p1, d.binary, d.op = outs[i].data_ptr(), av, bv, p0, p1, binary, code
+        return d
+    prog = [op(0, 0, U["neg"], x), op(1, 0, U["exp"], x), op(2, 0, U["log"], x, p0=1e-15),
+            op(3, 0, U["sqrt"], y), op(4, 0, U["sin"], y), op(5, 0, U["cos"], y), op(6, 0, U["tanh"], y),
+            op(7, 0, U["clamp"], y, p0=-1.0, p1=1.0), op(8, 0, U["scale"], x, p0=3.0),
+            op(9, 1, B["add"], x, y), op(10, 1, B["sub"], None, x, av=1.0), op(11, 1, B["mul"], x, None, bv=-4.0),
+            op(12, 1, B["div"], x, None, bv=0.0, p0=1e-15), op(13, 1, B["div"], y, x, p0=1e-15),
+            op(14, 1, B["mul"], outs[1], outs[9]),          # reads results made earlier in the program
+            op(15, 0, U["neg"], outs[14])]
+    L.scalar_program(prog, device=dev)
+    got = [float(t) for t in outs]
+    a, b = 0.37, -2.5
+    want = [-a, math.exp(a), math.log(a), 0.0, math.sin(b), math.cos(b), math.tanh(b), -1.0, 3 * a,
+            a + b, 1.0 - a, -4 * a, a / 1e-15, b / a, math.exp(a) * (a + b), -math.exp(a) * (a + b)]
+    for i, (g_, w) in enumerate(zip(got, want)):
+        assert g_ == pytest.approx(w, rel=1e-15, abs=0.0), (i, g_, w)
+
+
+def test_scalar_steps_between_levels_run_as_one_program():
+    """Scalar arithmetic on reduction results (neg, exp, log, mul by constants, clamp) between
+    fused levels: one device program, results against the FP64 oracle under the 2x rule."""
+    f = field((30, 26, 34), 7)
+    expr = "f * exp(-mean(f)) - log(norm_l2(f) + 1) * 0.5 + clamp(mean(f*f))"
+    run_and_check(expr, {"f": f}, reductions=("sum",))
