@@ -1860,6 +1860,27 @@ class KernelGen:
             A(f"{ind}        const unsigned vlo = (unsigned)vm, vhi = (unsigned)(vm >> 32);")
             A(f"{ind}        const IDX v0 = ((IDX){z} * N1 + (y - lane / {W})) * N2 + (x - lane % {W});")
             A(f"{ind}        wave_sync();")
+            # a wave whose 64 voxels were all evaluated here (the interior of the grid) stores
+            # without per-element tests: chunk q of 64 elements spans at most two rows (C * W >= 64),
+            # which are known at compile time, so only the chunk that straddles a row end picks
+            # the row per lane
+            full = all(self.info[self.group.outputs[oi]].ch * W >= 64 for oi in self.stage)
+            if full:
+                A(f"{ind}        if (vm == ~0ull) {{")
+                for oi in self.stage:
+                    C = self.info[self.group.outputs[oi]].ch
+                    A(f"{ind}          {{ const float* st = &so_stage[tid >> 6][{self.stage_off[oi]}];")
+                    A(f"{ind}            float* ob = (float*)p.out{oi} + v0 * {C};")
+                    for q in range(C):
+                        r_lo, r_hi = (64 * q // C) // W, ((64 * q + 63) // C) // W
+                        if r_lo == r_hi:
+                            off = f"{64 * q} + lane + {r_lo} * ((N2 - {W}) * {C})" if r_lo else f"{64 * q} + lane"
+                        else:
+                            thr = C * W * r_hi - 64 * q
+                            off = f"{64 * q} + lane + (lane >= {thr} ? {r_hi} : {r_lo}) * ((N2 - {W}) * {C})"
+                        A(f"{ind}            st_f32(ob, {off}, st[{64 * q} + lane]);")
+                    A(f"{ind}          }}")
+                A(f"{ind}        }} else {{")
             for oi in self.stage:
                 C = self.info[self.group.outputs[oi]].ch
                 A(f"{ind}        {{ const float* st = &so_stage[tid >> 6][{self.stage_off[oi]}];")
@@ -1870,6 +1891,8 @@ class KernelGen:
                 # the ballot's halves as 32-bit words: one register for the test, not two
                 A(f"{ind}            if (((v < 32 ? vlo : vhi) >> (v & 31)) & 1u) st_f32(ob, e + (v / {W}) * ((N2 - {W}) * {C}), st[e]);")
                 A(f"{ind}          }} }}")
+            if full:
+                A(f"{ind}        }}")
             A(f"{ind}        wave_sync();")
             A(f"{ind}      }}")
             A(f"{ind}    }}")
