@@ -33,10 +33,8 @@ __device__ __forceinline__ float det2_fma(float a, float b, float c, float d) {
 }
 
 __device__ __forceinline__ float clamp_det(float d) {
-    if (d != d) return 0.0f;
-    if (d > 1e30f) return 1e30f;
-    if (d < -1e30f) return -1e30f;
-    return d;
+    // NaN -> 0, then a clamp to [-1e30, 1e30] (one median-of-three on the GPU)
+    return d == d ? fminf(fmaxf(d, -1e30f), 1e30f) : 0.0f;
 }
 
 __device__ __forceinline__ float sym3_det_regularised(const float* m, float reg) {
@@ -64,14 +62,14 @@ __device__ __forceinline__ float sym3_det_regularised(const float* m, float reg)
 // the hardware reciprocal square root (1 ulp) refined by one Newton step in FP32,
 // r = r0 + r0 (1/2 - (s r0) r0 / 2), with the product s r0 rounded and the rest fused.  Each
 // component g_i r is then within about 2 ulp of g_i / max(|g|, eps), against about 1.5 ulp for the
-// FP32 formula that rounds the norm and the quotient.  Selects, not branches: 1 / eps below eps^2,
-// NaN for NaN, 0 for an infinite s (as the formula).
+// FP32 formula that rounds the norm and the quotient.  Below eps^2 the same step runs on eps^2
+// (r = 1 / eps to about 1 ulp); an infinite s gives 0 (as the formula); a NaN s gives NaN.
 __device__ __forceinline__ float unit_rnorm(float s, float eps) {
-    const float r0 = sym3_rsq(s);
-    const float e = fmaf(-0.5f * (s * r0), r0, 0.5f);
+    const float sc = s < eps * eps ? eps * eps : s;
+    const float r0 = sym3_rsq(sc);
+    const float e = fmaf(-0.5f * (sc * r0), r0, 0.5f);
     const float r = fmaf(r0, e, r0);
-    const float lo = s != s ? s : 1.0f / eps;
-    return s >= eps * eps ? (s > 3.4028234663852886e38f ? 0.0f : r) : lo;
+    return s > 3.4028234663852886e38f ? 0.0f : r;
 }
 
 // atan2(y, x) for y >= 0 (or NaN), the closed form's angle in [0, pi]: a = min / max of |x| and

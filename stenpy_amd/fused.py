@@ -118,6 +118,8 @@ SPLIT_TAIL = os.environ.get("STENPY_SPLIT_TAIL", "auto")
 MERGED_TAIL_MAX_VGPRS = 128
 # stage wide channel-last outputs through LDS so every store instruction writes whole lines
 STAGE_STORES = os.environ.get("STENPY_STAGE_STORES", "1") == "1"
+# the fewest channels an output needs for staging (a lane's C values are C * 4 bytes apart)
+STAGE_MIN_CH = int(os.environ.get("STENPY_STAGE_MIN_CH", "4"))
 COMBINE_MULTI_MAX = 16        # descriptors per sp_reduce_combine_multi launch
 BAND_MAX_FRACTION = 0.08
 
@@ -992,7 +994,7 @@ class KernelGen:
         # outputs of >= 4 channels written through a per-wave LDS transpose (coalesced lines)
         self.stage = [oi for oi, o in enumerate(group.outputs)
                       if STAGE_STORES and self.use_lds and not batched and (sched.tx % 64 == 0 or 64 % sched.tx == 0)
-                      and info[o].ch >= 4 and out_dtypes.get(o) != torch.bfloat16]
+                      and info[o].ch >= STAGE_MIN_CH and out_dtypes.get(o) != torch.bfloat16]
         # per wave: 64 * C floats per staged output, deposited by the lanes inside the variant
         self.stage_off: Dict[int, int] = {}
         self.stage_c = 0
