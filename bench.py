@@ -292,6 +292,56 @@ def bench_spectral(args) -> int:
     return 0
 
 
+def bench_plain(args) -> int:
+    """128^3 FP32 field (seed 1): the five single expressions of the plain eager workload, each
+    as one plain HIP kernel per node (the reference's eager executor, re-hosted) and fused, with
+    the FP64 CPU oracle's time and the error rule per expression."""
+    import torch
+    import stenpy_amd as sp
+    from stenpy_amd import workloads as W
+    from oracle import graph_eval
+    dev = torch.device("cuda:0")
+    from stenpy_amd import tune
+    rt = sp.Runtime(device=dev, fusion_enabled=True, schedule_fn=tune.schedule_fn(dev))
+    threads = cpu_threads()
+    torch.set_num_threads(threads)
+    steps = max(10, args.steps)
+    rows, t_fused, t_eager, t_cpu = {}, 0.0, 0.0, 0.0
+    for expr, g, wanted in W.plain_graphs(dev):
+        sink = wanted[0]
+        te = time_steps(lambda: rt.run_eager(g), steps, args.warmup)
+        tf = time_steps(lambda: rt.fused.run(g, rt, wanted=wanted), steps, args.warmup)
+        res = rt.fused.run(g, rt, wanted=wanted)
+        torch.cuda.synchronize()
+        field = [g.node(n).params["value"] for n in g._order if g.node(n).op_name == "_constant"
+                 and isinstance(g.node(n).params["value"], torch.Tensor) and g.node(n).params["value"].ndim == 3][0]
+        gc = sp.compile_expression(expr, dx=1.0, boundary="periodic", field_map={"f": field.cpu()})
+        g64, s64 = gc[0], gc[1]
+        t0 = time.perf_counter()
+        o64 = graph_eval.evaluate(g64, torch.float64)[s64]
+        tc = time.perf_counter() - t0
+        o32 = graph_eval.evaluate(g64, torch.float32)[s64]
+        b = res[sink].detach().double().cpu()
+        e_b = (b - o64.double()).abs().max().item()
+        e_32 = (o32.double() - o64.double()).abs().max().item()
+        rows[expr] = {"fused_ms": round(tf * 1e3, 4), "eager_ms": round(te * 1e3, 4),
+                      "cpu_oracle_ms": round(tc * 1e3, 2), "e_build": e_b, "e_ref32": e_32,
+                      "rule_2x": bool(e_b <= 2.0 * e_32)}
+        t_fused, t_eager, t_cpu = t_fused + tf, t_eager + te, t_cpu + tc
+    vox = 5 * 128 ** 3
+    _line("plain eager workload: Gvoxel/s over the five 128^3 expressions, fused",
+          round(vox / t_fused / 1e9, 3), "Gvoxel/s", steps, args.warmup, t_fused * 1e3,
+          {"workload": "128^3 FP32 Fourier field seed 1 (+ noise), periodic, dx = 1; "
+                       "f**2 + tanh(f)*exp(f), laplacian(f), gradient(f), mean(f), norm_l2(f), one at a time",
+           "expressions": rows, "eager_gvoxel_s": round(vox / t_eager / 1e9, 3),
+           "note": "a latency workload: the 8 MiB field stays in the 256 MB last-level cache"},
+          cpu_baseline={"value": round(vox / t_cpu / 1e9, 5), "unit": "Gvoxel/s", "cores": threads,
+                        "kind": "port", "sample": "the five expressions on the whole 128^3 field, FP64 torch CPU oracle",
+                        "cpu": cpu_model(),
+                        "note": "lower bound on the reference's CPU time (no executor gc / zero fill)"})
+    return 0
+
+
 def pinned_h2d_gbs(dev, nbytes: int = 1 << 30, reps: int = 5) -> float:
     """Host-to-device GB/s of one pinned 1 GiB buffer on this box (HIP events)."""
     import torch
@@ -472,8 +522,9 @@ def main() -> int:
     ap.add_argument("--tail", type=int, default=0,
                     help="with --stats-md: also report the mean of the last N fused_kernel dispatches")
     ap.add_argument("--out", type=str, default=None)
-    ap.add_argument("--workload", choices=["graph", "batched", "stream", "tensor", "spectral"], default="graph",
-                    help="graph: the 512^3 stencil graph (default); batched: 96 uneven grids x 6 "
+    ap.add_argument("--workload", choices=["graph", "plain", "batched", "stream", "tensor", "spectral"], default="graph",
+                    help="graph: the 512^3 stencil graph (default); plain: the five 128^3 "
+                         "expressions, plain and fused; batched: 96 uneven grids x 6 "
                          "random graphs in batched persistent launches; stream: out-of-core "
                          "4608x4096x4096 stream (bounded sample of slabs); tensor: 768^3 hessian "
                          "tensor ops; spectral: 512^3 spectral Laplacian")
@@ -490,8 +541,8 @@ def main() -> int:
     if args.profile_pass:
         return profile_pass(args)
     if args.workload != "graph":
-        return {"batched": bench_batched, "stream": bench_stream, "tensor": bench_tensor,
-                "spectral": bench_spectral}[args.workload](args)
+        return {"plain": bench_plain, "batched": bench_batched, "stream": bench_stream,
+                "tensor": bench_tensor, "spectral": bench_spectral}[args.workload](args)
 
     import torch
     import stenpy_amd as sp
