@@ -77,7 +77,13 @@ def main(argv: List[str] = None) -> int:
     o = out.double()
     summary.update({"min": float(o.min()), "max": float(o.max()), "mean": float(o.mean())})
     if a.perf_report:
-        summary["gvoxel_s"] = out.numel() / el / 1e9
+        summary["gvoxel_s"] = out.numel() / el / 1e9          # end to end, input loading included
+        if rt.last_path == "fused":
+            # the evaluation alone, re-run warm: HIP-event time, algorithmic bytes of the fused
+            # groups and the arithmetic of their generated code (stenpy.py:355-394 reports
+            # wall time and bytes; its flop counter is never incremented)
+            from stenpy_amd.profiling import fused_run_report
+            summary["perf"] = fused_run_report(rt, g, [sink])
     print(json.dumps(summary))
     return 0
 

@@ -113,3 +113,75 @@ def trace_tail(d: str, kernel: str, n: int) -> Dict[str, float]:
         raise KeyError(f"no dispatch of {kernel!r} under {d}")
     return {"dispatches": len(rows), "tail": len(tail), "mean_ns": sum(tail) / len(tail),
             "min_ns": min(tail), "max_ns": max(tail), "vgprs": rows[-1][2]}
+
+
+# ------------------------------------------------------------------------------ run reports
+HBM_PEAK_GBS = 8000.0
+# arithmetic operations of the device helpers the fused code calls (FP32 or FP64 operations
+# of their bodies in csrc/sp_sym3.h, csrc/sp_math32.h and csrc/fused_template.h, counted once)
+_CALL_OPS = {"tg_mid": 2, "tg_left2": 5, "tg_right2": 5, "unit_rnorm": 8, "div3f": 4,
+             "sym3_trace": 2, "sym3_det_regularised": 20, "sym3_deviatoric": 13,
+             "sym3_eigvalsh": 120, "mat3_inverse": 30, "spm_expf": 16, "spm_tanhf": 24,
+             "spm_logf": 18, "spm_sinf": 14, "spm_cosf": 14}
+
+
+def code_ops_per_voxel(lines: List[str]) -> int:
+    """Arithmetic operations per voxel of generated interior code: each +, -, *, / between
+    operands plus the operations of the helper functions it calls (``_CALL_OPS``)."""
+    import re
+    text = "\n".join(lines)
+    n = len(re.findall(r"(?<=[\w\)\]]) [-+*/] (?=[\w\(\-])", text))
+    for fn, k in _CALL_OPS.items():
+        n += k * len(re.findall(rf"\b{fn}\(", text))
+    return n
+
+
+def fused_run_report(rt, graph, wanted, reps: int = 3) -> Dict[str, object]:
+    """Time, traffic and arithmetic of one fused evaluation of ``graph`` on ``rt`` (plans are
+    compiled by a first run): HIP-event milliseconds of the whole evaluation, the algorithmic
+    bytes of its fused groups (every source channel read once, every materialised output
+    written once; fused reductions add nothing), the arithmetic operations of the generated
+    interior code, the achieved rates, and the rocprofv3 command that measures the same run's
+    HBM counters."""
+    import torch
+    from . import fused as F
+    results = rt.fused.run(graph, rt, wanted=wanted)
+    values = {n: results[n] for n in graph._order if graph.node(n).op_name == "_constant"}
+    steps, info, comp = rt.fused.prepare(graph, values, wanted)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        rt.fused.run(graph, rt, wanted=wanted)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    read = written = ops = voxels = 0
+    groups = []
+    for si, cg in comp.items():
+        gen = cg.gen
+        nvox = 1
+        for s in cg.group.grid:
+            nvox *= int(s)
+        r = sum(max(L_["C"], 1) for L_ in gen.src_layout) * 4 * nvox
+        w = sum(max(gen.info[o].ch, 1) * (2 if gen.out_dtypes.get(o) == torch.bfloat16 else 4)
+                for o in cg.group.outputs) * nvox
+        e = F._Emit(gen, ("lds", False, False)) if gen.use_lds else F._Emit(gen, ("gather",))
+        pos = e.origin()
+        for nid in cg.group.outputs:
+            e.val(nid, pos)
+        for nid in cg.group.reductions:
+            e.val(graph.node(nid).input_ids[0], pos)
+        k = code_ops_per_voxel(e.lines)
+        groups.append({"grid": list(cg.group.grid), "schedule": list(gen.sched.as_tuple()),
+                       "bytes_read": r, "bytes_written": w, "ops_per_voxel": k})
+        read, written, ops, voxels = read + r, written + w, ops + k * nvox, max(voxels, nvox)
+    s = ms * 1e-3
+    sig = lambda x: float(f"{x:.5g}")          # five significant digits
+    return {"gpu_ms": sig(ms), "voxels": voxels, "gvoxel_s": sig(voxels / s / 1e9),
+            "bytes_read": read, "bytes_written": written,
+            "achieved_gbs": sig((read + written) / s / 1e9),
+            "hbm_frac": sig((read + written) / s / 1e9 / HBM_PEAK_GBS),
+            "arith_ops": ops, "arith_gops": sig(ops / s / 1e9), "groups": groups,
+            "counters": "rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) --output-format csv -d DIR -o run -- "
+                        "<this command>: HBM bytes per kernel dispatch (KiB)"}

@@ -28,6 +28,13 @@ def test_cli_whole_field_from_npy(tmp_path, capsys):
     assert_2x(got, o64, o32, "cli " + expr)
     assert summary["shape"] == [16, 18, 20] and summary["gvoxel_s"] > 0
     assert abs(summary["mean"] - got.double().mean().item()) <= 1e-6
+    # the perf report: one fused group reading f once and writing the result once (4 B each per
+    # voxel), with the arithmetic of its generated code and rates from the measured time
+    perf = summary["perf"]
+    nvox = 16 * 18 * 20
+    assert perf["voxels"] == nvox and perf["bytes_read"] == 4 * nvox and perf["bytes_written"] == 4 * nvox
+    assert perf["gpu_ms"] > 0 and perf["arith_ops"] >= 10 * nvox and 0 < perf["hbm_frac"] < 1
+    assert perf["achieved_gbs"] == pytest.approx(8 * nvox / (perf["gpu_ms"] * 1e-3) / 1e9, rel=0.01)
 
 
 def test_cli_streamed_generator_field(tmp_path, capsys):

@@ -52,3 +52,11 @@ def test_kernel_stats_and_markdown(tmp_path):
     assert rows[0] == {"name": "fused_kernel", "calls": 20, "total_ns": 14.5e6, "avg_ns": 725e3, "pct": 90.0}
     md = profiling.stats_markdown(str(tmp_path), "cmd")
     assert md.splitlines()[4] == "| `fused_kernel` | 20 | 725.00 | 14.500 | 90.0 |"
+
+
+def test_code_ops_per_voxel_counts_operators_and_helper_calls():
+    from stenpy_amd.profiling import code_ops_per_voxel, _CALL_OPS
+    lines = ["const float v4 = ((((qp0[0] + qm0[0]) - 2.0f * q00[0]) / 1.0f) + (((v0 + v1) - 2.0f * q00[0]) / 1.0f));",
+             "const float v5 = tg_mid(a, b, 1.0f);", "float e47[3]; sym3_eigvalsh(t46, e47);",
+             "const float v6 = -v5;"]                   # a unary minus is not an operation between operands
+    assert code_ops_per_voxel(lines) == 9 + _CALL_OPS["tg_mid"] + _CALL_OPS["sym3_eigvalsh"]
