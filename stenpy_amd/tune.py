@@ -543,10 +543,28 @@ def config_groups(dev, which: Sequence[int] = (1, 2, 3, 4, 5)):
         yield "C5 768^3 tensor graph", g, sinks
 
 
+def score_rows(rows: List[Dict[str, Any]], model: "ScheduleModel", db: Optional[Dict[str, list]] = None,
+               log=None) -> Dict[str, Any]:
+    """``config_choice`` for every group of saved study rows (each row carries its
+    ``config`` label), with the summary."""
+    db = db or {}
+    by: Dict[Tuple[str, str], List[Dict[str, Any]]] = {}
+    for r in rows:
+        by.setdefault((r.get("config", ""), r["sig"]), []).append(r)
+    groups = []
+    for (label, sig), rs in by.items():
+        entry = {"config": label, **config_choice(rs, model, db.get(sig))}
+        groups.append(entry)
+        if log:
+            log(json.dumps({k: entry[k] for k in ("config", "best", "model_top1", "model_top3")}))
+    return {"groups": groups, "summary": summarize_choices(groups)}
+
+
 def config_regret(dev, which: Sequence[int] = (1, 2, 3, 4, 5), reps: int = 10, workers: int = 12,
-                  log=None) -> Dict[str, Any]:
+                  log=None, rows_out: Optional[List[Dict[str, Any]]] = None) -> Dict[str, Any]:
     """Exhaustive search over every candidate for each fused group of the benchmark
-    configurations, scored against the committed model's top-1 / top-3 picks."""
+    configurations, scored against the committed model's top-1 / top-3 picks.  The timing
+    rows are appended to ``rows_out`` when given (``score_rows`` re-scores them)."""
     from .runtime import Runtime
     model = load_default_model(dev)
     if model is None:
@@ -562,6 +580,8 @@ def config_regret(dev, which: Sequence[int] = (1, 2, 3, 4, 5), reps: int = 10, w
         by: Dict[str, List[Dict[str, Any]]] = {}
         for r in rows:
             by.setdefault(r["sig"], []).append(r)
+            if rows_out is not None:
+                rows_out.append({**r, "config": label})
         for sig, rs in by.items():
             entry = {"config": label, **config_choice(rs, model, db.get(sig))}
             report["groups"].append(entry)
@@ -645,6 +665,11 @@ def main(argv=None) -> int:
     r.add_argument("--which", default="1,2,3,4,5", help="comma-separated configuration numbers")
     r.add_argument("--out", default=None, help="write the JSON report here")
     r.add_argument("--workers", type=int, default=12)
+    r.add_argument("--save-rows", default=None, help="also write the timing rows (gzipped JSON lines)")
+    q = sub.add_parser("rescore", help="score a model against saved configuration-study rows")
+    q.add_argument("--rows", required=True)
+    q.add_argument("--model", default=str(MODEL_PATH))
+    q.add_argument("--out", default=None)
     a = ap.parse_args(argv)
 
     from .runtime import Runtime
@@ -673,12 +698,23 @@ def main(argv=None) -> int:
     if a.cmd == "configs":
         which = [int(v) for v in a.which.split(",") if v]
         t0 = time.time()
+        saved: List[Dict[str, Any]] = []
         rep = config_regret(dev, which, workers=a.workers,
-                            log=lambda m: print(f"[{time.time() - t0:.0f}s] {m}", flush=True))
+                            log=lambda m: print(f"[{time.time() - t0:.0f}s] {m}", flush=True), rows_out=saved)
+        if a.save_rows:
+            _append_rows(a.save_rows, saved)
         rep["command"] = "python -m stenpy_amd.tune configs --which " + a.which
         text = json.dumps(rep, indent=1) + "\n"
         if a.out:
             Path(a.out).write_text(text)
+        print(json.dumps(rep["summary"]), flush=True)
+        return 0
+    if a.cmd == "rescore":
+        model = ScheduleModel.load(a.model, dev)
+        rep = score_rows(_read_rows(a.rows), model, load_db())
+        rep["model"] = a.model
+        if a.out:
+            Path(a.out).write_text(json.dumps(rep, indent=1) + "\n")
         print(json.dumps(rep["summary"]), flush=True)
         return 0
     if a.cmd == "train":
