@@ -16,6 +16,7 @@ the same per-voxel semantics as the fused executor (and its parity tests).
 from __future__ import annotations
 
 import math
+import os
 import struct
 from dataclasses import dataclass, field as dc_field
 from typing import Any, Dict, List, Optional, Sequence, Tuple
@@ -28,7 +29,7 @@ from . import fused as F
 from .graph import Graph
 
 # block size shared by every structure of a batch module (one kernel, one block shape)
-BATCH_THREADS = 256
+BATCH_THREADS = int(os.environ.get("STENPY_BATCH_THREADS", "256"))
 _N_INT_FIELDS = 58            # ints after the pointers in a batched Params (see KernelGen._body)
 
 
