@@ -256,3 +256,27 @@ def test_scalar_steps_between_levels_run_as_one_program():
     f = field((30, 26, 34), 7)
     expr = "f * exp(-mean(f)) - log(norm_l2(f) + 1) * 0.5 + clamp(mean(f*f))"
     run_and_check(expr, {"f": f}, reductions=("sum",))
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (3, 4, 5), (2, 5, 6), (1, 6, 7), (4, 1, 5)])
+@pytest.mark.parametrize("expr", ["f**2 + tanh(f)*exp(f)", "laplacian(f)", "gradient(f)",
+                                  "divergence(gradient(f)) + mean(f)", "norm_l2(f) * f"])
+def test_small_and_degenerate_grids(shape, expr):
+    """Grids at the fused path's lower limit (every axis 3) and below it (an axis of 1 or 2,
+    which the plain per-op kernels take), periodic: the runtime's result against the oracle."""
+    f = field(shape, 23)
+    g, sink, *_ = sp.compile_expression(expr, boundary="periodic", field_map={"f": f})
+    rt = sp.Runtime(device="cuda", fusion_enabled=True)
+    out = rt.run(g)[sink]
+    o64 = graph_eval.evaluate(g, torch.float64)[sink]
+    o32 = graph_eval.evaluate(g, torch.float32)[sink]
+    assert tuple(out.shape) == tuple(o64.shape)
+    assert_2x(out, o64, o32, f"{expr} on {shape}")
+
+
+def test_empty_field_point_wise():
+    """A field with no voxels evaluates to an empty result, as torch's element-wise ops give."""
+    f = torch.empty((0, 4, 4), device="cuda")
+    g, sink, *_ = sp.compile_expression("f**2 + tanh(f)*exp(f)", boundary="periodic", field_map={"f": f})
+    out = sp.Runtime(device="cuda", fusion_enabled=True).run(g)[sink]
+    assert tuple(out.shape) == (0, 4, 4)
