@@ -2198,7 +2198,7 @@ class FusedExecutor:
             if st.kind == "eager":
                 node = graph.node(st.node)
                 ins = [results[i] for i in node.input_ids]
-                if info[st.node].ch == 0 and node.op_name not in REDUCTION_OPS:
+                if info[st.node].ch == 0 and (node.op_name in _SCALAR_UNARY or node.op_name in _SCALAR_BINARY):
                     out = torch.empty((), dtype=torch.float64, device=self.device)
                     d = scalar_op(node, ins, out)
                     if d is not None:
