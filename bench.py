@@ -245,6 +245,7 @@ def bench_tensor(args) -> int:
         tj = json.loads(tp.read_text())
         if tj.get("n") == n and tuple(tj.get("schedule", ())) == tuple(cg.gen.sched.as_tuple()):
             traffic = round(tj["bytes_per_launch"])
+    parity = tensor_slab_parity(f, res, sinks) if not args.no_cpu else None
     _line("tensor ops: Gvoxel/s of hessian -> eig/det/trace/dev + mean curvature on 768^3",
           round(nvox / t / 1e9, 3), "Gvoxel/s", max(3, args.steps // 4), 1, t * 1e3,
           {"workload": f"{n}^3 FP32, lower half Fourier seed 8, upper half i^2+j^2+k^2; periodic; "
@@ -256,9 +257,35 @@ def bench_tensor(args) -> int:
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel_ms": round(k_ms, 4), "algorithmic_bytes_per_launch": alg,
                     "kernels": "fused_flag (+ fused_flag_tail), fused_kernel, fused_tail (side stream)",
-                    "note": "VALU issue is the co-limit: counters give ~68% of the kernel's cycles "
-                            "issuing VALU (profiles/tensor_768_fused_kernel_counters.md)"})
+                    "note": "VALU issue is the co-limit: counters give ~63% of the kernel's cycles "
+                            "issuing VALU (profiles/tensor_768_fused_kernel_counters.md)"},
+          parity=parity)
     return 0
+
+
+def tensor_slab_parity(f, res, sinks, z0: int = 380, z1: int = 388, margin: int = 3):
+    """The 2x error rule on planes [z0, z1) of the measured run's five outputs, across the seam
+    between the Fourier half and i^2 + j^2 + k^2: the FP64 oracle and its FP32 twin evaluate the
+    same graph on planes [z0 - margin, z1 + margin) of the input, whose interior planes equal the
+    whole field's (the stencils reach 2 planes).  The oracle decides the determinant's global
+    +1e-12 I from the slab alone; the shift it could miss is far below the FP32 errors."""
+    import torch
+    from oracle import graph_eval
+    from stenpy_amd.compiler import compile_expressions
+    from stenpy_amd.workloads import TENSOR_EXPRS
+    torch.set_num_threads(cpu_threads())
+    slab = f[z0 - margin:z1 + margin].cpu()
+    g, s_, *_ = compile_expressions(TENSOR_EXPRS, boundary="periodic", field_map={"f": slab})
+    o64 = graph_eval.evaluate(g, torch.float64)
+    o32 = graph_eval.evaluate(g, torch.float32)
+    out = {"planes": [z0, z1], "rule": "e_build <= 2 e_ref32 against the FP64 oracle"}
+    for expr, nb, no in zip(TENSOR_EXPRS, sinks, s_):
+        b = res[nb][z0:z1].detach().double().cpu()
+        ref = o64[no][margin:margin + (z1 - z0)].double()
+        e_b = (b - ref).abs().max().item()
+        e_32 = (o32[no][margin:margin + (z1 - z0)].double() - ref).abs().max().item()
+        out[expr] = {"e_build": e_b, "e_ref32": e_32, "rule_2x": bool(e_b <= 2.0 * e_32)}
+    return out
 
 
 def bench_spectral(args) -> int:
