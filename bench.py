@@ -181,11 +181,15 @@ def bench_batched(args) -> int:
     dev = torch.device("cuda:0")
     exprs, items = c3_items(dev)
     t0 = time.perf_counter()
-    # default: the batch's own shape heuristic.  The model's picks, ranked on single-field
-    # timings, favour short z-chunks that cost more inside one shared persistent launch (measured
-    # 217 against 247 Gvoxel/s); ``--schedules model`` runs them (one block size per module).
-    use_model = args.schedules == "model"
-    sfn = tune.schedule_fn(dev, threads=B.BATCH_THREADS) if use_model else None
+    # default: the per-structure winners of the batch autotuner (python -m stenpy_amd.tune batch)
+    # from the tuning database, the batch's shape heuristic for structures it has none for.  The
+    # model's picks, ranked on single-field timings, favour short z-chunks that cost more inside
+    # one shared persistent launch (measured 217 against 247 Gvoxel/s); ``--schedules model``
+    # runs them (one block size per module).
+    src = {"model": "the learned model", "heuristic": "batch shape heuristic"}.get(
+        args.schedules, "batch autotuner (tuning database), else the batch shape heuristic")
+    sfn = {"model": lambda: tune.schedule_fn(dev, threads=B.BATCH_THREADS), "heuristic": lambda: None}.get(
+        args.schedules, lambda: tune.batch_schedule_fn(B.BATCH_THREADS))()
     br = B.BatchRunner(items, device=dev, schedule_fn=sfn)
     br.compile(workers=8)
     t_plan = time.perf_counter() - t0
@@ -205,7 +209,7 @@ def bench_batched(args) -> int:
           {"workload": "96 fields, axes U{32..384} seed 6; 6 random graphs (3-12 ops) seed 6; periodic",
            "exprs": exprs, "levels": st["levels"], "units": st["units"], "structures": st["structures"],
            "voxels": vox, "plan_compile_s": round(t_plan, 2),
-           "schedule_source": "tuning database, then the learned model" if use_model else "batch shape heuristic",
+           "schedule_source": src,
            "schedules": br.schedules(),
            "sequential_fused_ms": round(t_seq * 1e3, 3), "speedup_vs_sequential": round(t_seq / t, 2)},
           roofline={"bound": "hbm", "achieved": round(st["algorithmic_bytes"] / t / 1e9, 1),
@@ -556,8 +560,9 @@ def main() -> int:
                          "4608x4096x4096 stream (bounded sample of slabs); tensor: 768^3 hessian "
                          "tensor ops; spectral: 512^3 spectral Laplacian")
     ap.add_argument("--schedules", choices=["auto", "heuristic", "model"], default="auto",
-                    help="batched / stream schedule source; auto: the batch heuristic for batched, "
-                         "the tuning database then the learned model for stream")
+                    help="batched / stream schedule source; auto: the batch autotuner's database entries "
+                         "(else the batch heuristic) for batched, the tuning database then the learned "
+                         "model for stream")
     ap.add_argument("--slabs", type=int, default=0, help="stream: slabs of 256 planes to stream (0: all 18)")
     args = ap.parse_args()
     if args.pmc_traffic or args.stats_md:

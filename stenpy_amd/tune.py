@@ -142,6 +142,106 @@ def schedule_fn(device: Optional[torch.device] = None, use_model: bool = True,
     return pick
 
 
+# ------------------------------------------------------------------------------ batched launches
+# per-structure schedules of a batched launch (one block size per batch module)
+BATCH_CANDIDATES: List[Schedule] = [Schedule(tx, ty, zc, 256) for tx, ty in ((64, 8), (64, 4), (32, 16), (32, 8))
+                                    for zc in (64, 128, 256)]
+
+
+def _batch_key(structure: str, threads: int) -> str:
+    return f"batch|threads={threads}|{structure}"
+
+
+def batch_schedule_fn(threads: int = 256):
+    """Schedule source for ``BatchRunner``: the per-structure winners ``batch_autotune`` stored
+    in the tuning database, else (by returning None) the batch's shape heuristic."""
+    from .batch import structure_key
+    db = load_db()
+
+    def pick(group, graph) -> Optional[Schedule]:
+        hit = db.get(_batch_key(structure_key(group, graph), threads))
+        return Schedule(*hit) if hit else None
+
+    return pick
+
+
+def batch_step_ms(runner, steps: int = 20, warmup: int = 3) -> float:
+    """Milliseconds per replay of the runner's captured batch (HIP events)."""
+    g = runner.capture()
+    for _ in range(warmup):
+        g.replay()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        g.replay()
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def batch_autotune(items, device, cands: Sequence[Schedule] = None, min_gain: float = 0.01,
+                   update_db: bool = True, log=None) -> Dict[str, Any]:
+    """Greedy per-structure search for a batched launch, timed on the whole batch.
+
+    Starts from the batch heuristic for every structure; for each structure in turn, tries
+    every candidate with the others fixed and keeps one that shortens the replayed step by more
+    than ``min_gain``.  The result can only match or beat the heuristic on this batch.  Winners
+    are stored in the tuning database under the structure's key (``batch_schedule_fn``), the
+    structures the search left on the heuristic's schedule included."""
+    from .batch import BATCH_THREADS, BatchRunner
+    cands = list(cands or [c for c in BATCH_CANDIDATES if c.threads == BATCH_THREADS])
+    chosen: Dict[str, Schedule] = {}
+
+    def measure() -> Tuple[float, Any]:
+        fn = lambda grp, gr: chosen.get(_structure(grp, gr))
+        br = BatchRunner(items, device=device, schedule_fn=fn)
+        br.compile(workers=8)
+        ms = batch_step_ms(br)
+        return ms, br
+
+    from .batch import structure_key as _structure
+    best, br = measure()
+    base = best
+    keys = [k for m in br.mods for k in m["keys"]]
+    start = {k: tuple(g.sched.as_tuple()) for m in br.mods for k, g in zip(m["keys"], m["gens"])}
+    del br
+    if log:
+        log(f"heuristic: {base:.4f} ms, {len(keys)} structures")
+    trials = []
+    for k in keys:
+        for c in cands:
+            if c.as_tuple() == start[k] and k not in chosen:
+                continue
+            prev = chosen.get(k)
+            chosen[k] = c
+            try:
+                ms, br = measure()
+                del br
+            except (RuntimeError, ValueError) as exc:        # a schedule the structure cannot take
+                ms = float("inf")
+                if log:
+                    log(f"  {c.as_tuple()} skipped: {exc}")
+            trials.append({"structure": k[:80], "schedule": list(c.as_tuple()), "ms": ms})
+            if ms < best * (1.0 - min_gain):
+                best = ms
+                if log:
+                    log(f"  {k[:60]}: {c.as_tuple()} -> {ms:.4f} ms (kept)")
+            else:
+                if prev is None:
+                    del chosen[k]
+                else:
+                    chosen[k] = prev
+        torch.cuda.empty_cache()
+    final = {k: (list(chosen[k].as_tuple()) if k in chosen else list(start[k])) for k in keys}
+    if update_db:
+        db = load_db()
+        for k, c in final.items():
+            db[_batch_key(k, BATCH_THREADS)] = c
+        save_db(db)
+    return {"heuristic_ms": base, "best_ms": best, "changed": {k[:80]: final[k] for k in chosen},
+            "schedules": {k[:80]: c for k, c in final.items()}, "trials": trials}
+
+
 # ------------------------------------------------------------------------------ features
 def group_stats(gen: F.KernelGen) -> Dict[str, float]:
     """Static statistics of a group's generated interior code (schedule-independent)."""
@@ -666,6 +766,8 @@ def main(argv=None) -> int:
     r.add_argument("--out", default=None, help="write the JSON report here")
     r.add_argument("--workers", type=int, default=12)
     r.add_argument("--save-rows", default=None, help="also write the timing rows (gzipped JSON lines)")
+    bt = sub.add_parser("batch", help="greedy per-structure schedule search for the 96-grid batch")
+    bt.add_argument("--out", default=None)
     q = sub.add_parser("rescore", help="score a model against saved configuration-study rows")
     q.add_argument("--rows", required=True)
     q.add_argument("--model", default=str(MODEL_PATH))
@@ -708,6 +810,15 @@ def main(argv=None) -> int:
         if a.out:
             Path(a.out).write_text(text)
         print(json.dumps(rep["summary"]), flush=True)
+        return 0
+    if a.cmd == "batch":
+        from . import workloads as W
+        t0 = time.time()
+        _, items = W.batched_items(dev)
+        rep = batch_autotune(items, dev, log=lambda m: print(f"[{time.time() - t0:.0f}s] {m}", flush=True))
+        if a.out:
+            Path(a.out).write_text(json.dumps(rep, indent=1) + "\n")
+        print(json.dumps({k: rep[k] for k in ("heuristic_ms", "best_ms", "changed")}), flush=True)
         return 0
     if a.cmd == "rescore":
         model = ScheduleModel.load(a.model, dev)
