@@ -114,14 +114,16 @@ def oracle_leg(f, u, res, sink, reds):
 def hbm_ceilings(f, u, reps: int = 10):
     """What this GPU sustains for three read:write mixes, from the library's streaming probes
     on the benchmark's own buffers (f: n^3 FP32, u: n^3 x 3): copy of f (1:1), the stencil
-    graph's mix (read f and u, write one float per voxel: 16 B read, 4 B written, 4:1) and a
-    read-only pass over u.  HIP events on the launch stream; the best of the flat and the
+    graph's mix (read f and u, write one float per voxel: 16 B read, 4 B written, 4:1; u read
+    as three planar thirds, and channel-last as the graph holds it), a read-only pass over u and
+    a write-only pass.  HIP events on the launch stream; the best of the flat and the
     persistent launch shape, each the mean of ``reps`` back-to-back launches.  GB/s."""
     import torch
     from stenpy_amd import _lib as L
     n = f.numel()
     out = torch.empty_like(f)
     cases = {"copy": (8 * n, dict(a=f, out=out)), "4r1w": (20 * n, dict(a=f, u=u, out=out)),
+             "4r1w_aos": (20 * n, dict(a=f, u=u, out=out)),
              "read": (12 * n, dict(a=u)), "write": (4 * n, dict(a=out))}
     res = {}
     for mix, (nbytes, kw) in cases.items():
@@ -337,11 +339,14 @@ def bench_plain(args) -> int:
     threads = cpu_threads()
     torch.set_num_threads(threads)
     steps = max(10, args.steps)
-    rows, t_fused, t_eager, t_cpu = {}, 0.0, 0.0, 0.0
+    rows, t_fused, t_graph, t_eager, t_cpu = {}, 0.0, 0.0, 0.0, 0.0
     for expr, g, wanted in W.plain_graphs(dev):
         sink = wanted[0]
         te = time_steps(lambda: rt.run_eager(g), steps, args.warmup)
         tf = time_steps(lambda: rt.fused.run(g, rt, wanted=wanted), steps, args.warmup)
+        cap, _ = rt.fused.capture(g, rt, wanted)
+        tg = time_steps(cap.replay, steps, args.warmup)
+        del cap
         res = rt.fused.run(g, rt, wanted=wanted)
         torch.cuda.synchronize()
         field = [g.node(n).params["value"] for n in g._order if g.node(n).op_name == "_constant"
@@ -355,16 +360,18 @@ def bench_plain(args) -> int:
         b = res[sink].detach().double().cpu()
         e_b = (b - o64.double()).abs().max().item()
         e_32 = (o32.double() - o64.double()).abs().max().item()
-        rows[expr] = {"fused_ms": round(tf * 1e3, 4), "eager_ms": round(te * 1e3, 4),
+        rows[expr] = {"fused_ms": round(tf * 1e3, 4), "fused_graph_replay_ms": round(tg * 1e3, 4),
+                      "eager_ms": round(te * 1e3, 4),
                       "cpu_oracle_ms": round(tc * 1e3, 2), "e_build": e_b, "e_ref32": e_32,
                       "rule_2x": bool(e_b <= 2.0 * e_32)}
-        t_fused, t_eager, t_cpu = t_fused + tf, t_eager + te, t_cpu + tc
+        t_fused, t_graph, t_eager, t_cpu = t_fused + tf, t_graph + tg, t_eager + te, t_cpu + tc
     vox = 5 * 128 ** 3
     _line("plain eager workload: Gvoxel/s over the five 128^3 expressions, fused",
           round(vox / t_fused / 1e9, 3), "Gvoxel/s", steps, args.warmup, t_fused * 1e3,
           {"workload": "128^3 FP32 Fourier field seed 1 (+ noise), periodic, dx = 1; "
                        "f**2 + tanh(f)*exp(f), laplacian(f), gradient(f), mean(f), norm_l2(f), one at a time",
            "expressions": rows, "eager_gvoxel_s": round(vox / t_eager / 1e9, 3),
+           "fused_graph_replay_gvoxel_s": round(vox / t_graph / 1e9, 3),
            "note": "a latency workload: the 8 MiB field stays in the 256 MB last-level cache"},
           cpu_baseline={"value": round(vox / t_cpu / 1e9, 5), "unit": "Gvoxel/s", "cores": threads,
                         "kind": "port", "sample": "the five expressions on the whole 128^3 field, FP64 torch CPU oracle",
@@ -649,12 +656,14 @@ def main() -> int:
                    "graph_replay_ms_per_step": round(t_replay * 1e3, 4),
                    "bf16_graph_replay_gvoxel_s": round(nvox / t_bf16_replay / 1e9, 2),
                    "probe_gbs": {"copy_1r1w": ceil["copy"], "stencil_mix_4r1w": ceil["4r1w"],
+                                 "stencil_mix_4r1w_channel_last": ceil["4r1w_aos"],
                                  "read_only": ceil["read"], "write_only": ceil["write"]},
                    "bf16_out_hbm_frac_step": round(18 * nvox / t_bf16 / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel_ms": round(k_ms, 4),
                      "frac_of_probe_4r1w": round(achieved / ceil["4r1w"], 4),
+                     "frac_of_probe_4r1w_channel_last": round(achieved / ceil["4r1w_aos"], 4),
                      "kernels": ("fused_kernel (LDS z-march, face wave) + fused_tail (edge band, side stream)"
                                  if cg.gen.split_tail else
                                  "fused_kernel (LDS z-march, face wave, edge-band blocks at the end of the grid)"),

@@ -297,12 +297,13 @@ int sp_cast_f32_bf16(const float* x, int64_t n, void* y, void* stream);
 /* HBM streaming probes that measure what this GPU sustains for a read:write mix, reported
  * beside the 8 TB/s roofline.  COPY: out = a (1:1); 4R1W: reads a (n floats) and u (3n floats,
  * as three coalesced thirds), writes out (n floats), the stencil graph's 16 B read / 4 B written
- * per voxel;
+ * per voxel; 4R1W_AOS: the same mix with u channel-last (3 floats per voxel, the graph's layout);
  * READ: one double per block (sum of its elements) into ``partials``; WRITE: fills ``a`` (n
  * floats) with values that depend on the index, writes only.  ``persistent`` = 0
  * launches one float4 group per thread; 1 launches 8 blocks per CU that stride over the range.
  * n must be a multiple of 4 and every pointer 16-byte aligned. */
-enum { SP_PROBE_COPY = 0, SP_PROBE_4R1W = 1, SP_PROBE_READ = 2, SP_PROBE_WRITE = 3 };
+enum { SP_PROBE_COPY = 0, SP_PROBE_4R1W = 1, SP_PROBE_READ = 2, SP_PROBE_WRITE = 3,
+       SP_PROBE_4R1W_AOS = 4 };
 int     sp_probe_stream(int mix, int persistent, const float* a, const float* u, float* out,
                         double* partials, int64_t n, int n_cu, void* stream);
 /* Blocks (and so ``partials`` entries) that sp_probe_stream launches for these arguments. */

@@ -465,7 +465,7 @@ def host_gen_fourier(out_ptr: int, shape, z0: int, nz: int, ks: np.ndarray, amps
         raise ValueError("host_gen_fourier: bad arguments")
 
 
-PROBE = {"copy": 0, "4r1w": 1, "read": 2, "write": 3}
+PROBE = {"copy": 0, "4r1w": 1, "read": 2, "write": 3, "4r1w_aos": 4}
 
 
 def probe_stream(mix: str, persistent: bool, a: torch.Tensor, u: Optional[torch.Tensor] = None,

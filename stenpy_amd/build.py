@@ -25,6 +25,8 @@ ARCH = os.environ.get("STENPY_OFFLOAD_ARCH", "gfx950")
 COMMON_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FMA contraction except where a function's pragma forbids it (compensated sums)
+    "-ffp-contract=fast-honor-pragmas",
     "-Wno-unused-command-line-argument",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]

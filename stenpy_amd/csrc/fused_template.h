@@ -59,20 +59,31 @@ __device__ __forceinline__ void st_f32(float* p, i64 i, float v) { __builtin_non
 __device__ __forceinline__ void st_bf16(unsigned short* p, i64 i, float v) { p[i] = f2bf(v); }
 
 // ---- compensated accumulators ----------------------------------------------------------------
+// Each function that forms an error term turns FMA contraction off: contracting x * k into the
+// running sum (t = fma(x, k, s)) would leave t without the separately rounded product that the
+// two-sum and the FMA split of the product assume.
 struct RSum {  // Neumaier sum
     float s, c;
     __device__ __forceinline__ void init() { s = 0.f; c = 0.f; }
     __device__ __forceinline__ void add(float x) {
+#pragma clang fp contract(off)
         float t = s + x;
         float bb = t - s;
         c += (s - (t - bb)) + (x - bb);
         s = t;
     }
 };
+__device__ __forceinline__ void rsum_add_prod(RSum& a, float x, float k) {  // a += x*k exactly split
+#pragma clang fp contract(off)
+    const float q = x * k;
+    a.add(q);
+    a.c += fmaf(x, k, -q);
+}
 struct RVar {  // compensated first and second moments (exact squares via FMA)
     RSum s1, s2;
     __device__ __forceinline__ void init() { s1.init(); s2.init(); }
     __device__ __forceinline__ void add(float x) {
+#pragma clang fp contract(off)
         s1.add(x);
         const float q = x * x;
         s2.add(q);
@@ -80,6 +91,7 @@ struct RVar {  // compensated first and second moments (exact squares via FMA)
     }
 };
 __device__ __forceinline__ void rsum_add_sq(RSum& a, float x) {  // a += x*x exactly split
+#pragma clang fp contract(off)
     const float q = x * x;
     a.add(q);
     a.c += fmaf(x, x, -q);
@@ -94,6 +106,7 @@ struct RMinMax {
 };
 
 __device__ __forceinline__ void merge_sum(RSum& a, float s, float c) {
+#pragma clang fp contract(off)
     float t = a.s + s;
     float bb = t - a.s;
     float e = (a.s - (t - bb)) + (s - bb);
@@ -108,7 +121,7 @@ __device__ __forceinline__ void merge_mm(RMinMax& a, float lo, float hi) {
 
 // Block-wide reductions; ``scratch`` holds at least blockDim.x/64 * 4 floats.  Thread 0 of
 // the block writes the 4-double record.
-__device__ __forceinline__ void block_sum(RSum a, float* scratch, double* rec) {
+__device__ __forceinline__ void block_sum(RSum a, float* scratch, double* rec, double scale = 1.0) {
     for (int m = 32; m >= 1; m >>= 1) merge_sum(a, __shfl_xor(a.s, m), __shfl_xor(a.c, m));
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     __syncthreads();
@@ -117,7 +130,7 @@ __device__ __forceinline__ void block_sum(RSum a, float* scratch, double* rec) {
     if (threadIdx.x == 0) {
         RSum r; r.s = scratch[0]; r.c = scratch[1];
         for (int i = 1; i < (int)(blockDim.x >> 6); ++i) merge_sum(r, scratch[2 * i], scratch[2 * i + 1]);
-        rec[0] = r.s; rec[1] = r.c; rec[2] = 0.0; rec[3] = 0.0;
+        rec[0] = (double)r.s * scale; rec[1] = (double)r.c * scale; rec[2] = 0.0; rec[3] = 0.0;
     }
 }
 __device__ __forceinline__ void block_var(RVar a, float* scratch, double* rec) {
@@ -155,21 +168,20 @@ __device__ __forceinline__ void block_mm(RMinMax a, float* scratch, double* rec)
     }
 }
 
-// Per-axis quadrature weight (Simpson with a trailing trapezoid for even n; trapezoid for
-// n < 3 or method 1), the weights of stenpy_engine.py:2415-2457 applied separably.
-__device__ __forceinline__ float quad_weight(int i, int n, float dx, int method) {
+// Per-axis quadrature weight as an integer numerator k over 6, the weight being k * dx / 6
+// (Simpson with a trailing trapezoid for even n; trapezoid for n < 3 or method 1), the weights
+// of stenpy_engine.py:2415-2457 applied separably.  The product of three axes' numerators (at
+// most 8^3) is exact in FP32; dx0 dx1 dx2 / 216 multiplies the block record once, in FP64.
+__device__ __forceinline__ float quad_k(int i, int n, int method) {
     if (method == 1 || n < 3) {
         if (n < 2) return 0.f;
-        return (i == 0 || i == n - 1) ? 0.5f * dx : dx;
+        return (i == 0 || i == n - 1) ? 3.f : 6.f;
     }
     const int m = (n % 2 == 0) ? n - 1 : n;
-    float w = 0.f;
-    if (i < m) {
-        if (i == 0 || i == m - 1) w = dx / 3.f;
-        else w = (i % 2 == 1) ? 4.f * dx / 3.f : 2.f * dx / 3.f;
-    }
-    if (n % 2 == 0 && i >= n - 2) w += 0.5f * dx;
-    return w;
+    float k = 0.f;
+    if (i < m) k = (i == 0 || i == m - 1) ? 2.f : ((i % 2 == 1) ? 8.f : 4.f);
+    if (n % 2 == 0 && i >= n - 2) k += 3.f;
+    return k;
 }
 
 // torch.gradient one-sided formulas (interior ((x+)-(x-))/h/2)

@@ -7,6 +7,10 @@
 //                   (16 B read, 4 B written per voxel).  u is read as three contiguous thirds
 //                   so that every load is fully coalesced: the probe measures the mix, not an
 //                   access pattern
+//   SP_PROBE_4R1W_AOS the same mix with u channel-last (voxel-major, 3 floats per voxel), the
+//                   graph's own layout: a thread's four voxels read 48 contiguous bytes of u as
+//                   three 16-byte loads, so each load instruction covers a third of its wave's
+//                   3 KiB and the three together cover it whole
 //   SP_PROBE_READ   per-block sums of a[i]                 reads only (one double per block)
 //   SP_PROBE_WRITE  out[i] = i-dependent constant           writes only
 //
@@ -47,6 +51,15 @@ __global__ void __launch_bounds__(kThreads) k_4r1w(const float4* __restrict__ f,
     const int64_t stride = persistent ? (int64_t)gridDim.x * kThreads : n4;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += stride)
         o[i] = mix4(f[i], u[i], u[n4 + i], u[2 * n4 + i]);
+}
+
+__global__ void __launch_bounds__(kThreads) k_4r1w_aos(const float4* __restrict__ f,
+                                                       const float4* __restrict__ u,
+                                                       float4* __restrict__ o, int64_t n4,
+                                                       int persistent) {
+    const int64_t stride = persistent ? (int64_t)gridDim.x * kThreads : n4;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n4; i += stride)
+        o[i] = mix4(f[i], u[3 * i], u[3 * i + 1], u[3 * i + 2]);
 }
 
 __global__ void __launch_bounds__(kThreads) k_read(const float4* __restrict__ a,
@@ -104,6 +117,12 @@ extern "C" int sp_probe_stream(int mix, int persistent, const float* a, const fl
             SP_REQUIRE(out && u && ((uintptr_t)out & 15) == 0 && ((uintptr_t)u & 15) == 0,
                        "probe: u and out must be 16-byte aligned");
             hipLaunchKernelGGL(k_4r1w, dim3(g), dim3(kThreads), 0, s, (const float4*)a,
+                               (const float4*)u, (float4*)out, n4, persistent);
+            break;
+        case SP_PROBE_4R1W_AOS:
+            SP_REQUIRE(out && u && ((uintptr_t)out & 15) == 0 && ((uintptr_t)u & 15) == 0,
+                       "probe: u and out must be 16-byte aligned");
+            hipLaunchKernelGGL(k_4r1w_aos, dim3(g), dim3(kThreads), 0, s, (const float4*)a,
                                (const float4*)u, (float4*)out, n4, persistent);
             break;
         case SP_PROBE_WRITE:

@@ -27,7 +27,10 @@ struct Acc {
     float lo, hi;     // min / max
 };
 
+// The compensated helpers turn FMA contraction off: a product contracted into the running sum
+// would leave no separately rounded term for the two-sum and the FMA split to correct.
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+#pragma clang fp contract(off)
     s = a + b;
     float bb = s - a;
     e = (a - (s - bb)) + (b - bb);
@@ -35,18 +38,27 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
 
 // a.s/a.c += x and a.s2/a.c2 += x^2, the square split exactly with an FMA
 __device__ __forceinline__ void add_sum(float& s, float& c, float x) {
+#pragma clang fp contract(off)
     float t, e;
     two_sum(s, x, t, e);
     s = t;
     c += e;
 }
 __device__ __forceinline__ void add_sq(float& s, float& c, float x) {
+#pragma clang fp contract(off)
     const float q = x * x;
     add_sum(s, c, q);
     c += fmaf(x, x, -q);
 }
+__device__ __forceinline__ void add_prod(float& s, float& c, float x, float w) {
+#pragma clang fp contract(off)
+    const float q = x * w;
+    add_sum(s, c, q);
+    c += fmaf(x, w, -q);
+}
 
 __device__ __forceinline__ void merge(Acc& a, const Acc& b) {
+#pragma clang fp contract(off)
     float s, e;
     two_sum(a.s, b.s, s, e);
     a.s = s;
@@ -69,25 +81,25 @@ __device__ __forceinline__ Acc shfl_acc(const Acc& a, int lane_mask) {
     return b;
 }
 
-// Per-axis quadrature weight of index i on an axis of n points (reference Simpson/trapezoid).
-__device__ __forceinline__ double quad_w(int64_t i, int64_t n, double dx, int method) {
+// Per-axis quadrature weight of index i on an axis of n points (reference Simpson/trapezoid,
+// stenpy_engine.py:2415-2457) as an integer numerator k over 6: the weight is k * dx / 6.  The
+// product of three numerators (at most 8^3) is exact in FP32, each term x * k is split exactly
+// with an FMA, and dx0 dx1 dx2 / 216 multiplies the block record once, in FP64.
+__device__ __forceinline__ float quad_k(int64_t i, int64_t n, int method) {
     if (method == 1 || n < 3) {  // trapezoid (also Simpson's n < 3 branch)
-        if (n < 2) return 0.0;
-        return (i == 0 || i == n - 1) ? 0.5 * dx : dx;
+        if (n < 2) return 0.f;
+        return (i == 0 || i == n - 1) ? 3.f : 6.f;
     }
     const int64_t m = (n % 2 == 0) ? n - 1 : n;  // Simpson points
-    double w = 0.0;
-    if (i < m) {
-        if (i == 0 || i == m - 1) w = dx / 3.0;
-        else w = (i % 2 == 1) ? 4.0 * dx / 3.0 : 2.0 * dx / 3.0;
-    }
-    if (n % 2 == 0 && i >= n - 2) w += 0.5 * dx;  // trapezoid on the last interval
-    return w;
+    float k = 0.f;
+    if (i < m) k = (i == 0 || i == m - 1) ? 2.f : ((i % 2 == 1) ? 8.f : 4.f);
+    if (n % 2 == 0 && i >= n - 2) k += 3.f;  // trapezoid on the last interval
+    return k;
 }
 
 struct IntegrateGeom {
     int64_t n0, n1, n2;
-    double dx0, dx1, dx2;
+    double scale;  // dx0 dx1 dx2 / 216
     int method;
 };
 
@@ -109,10 +121,9 @@ __global__ void __launch_bounds__(kThreads) k_reduce_stage1(int kind, const floa
             const int64_t r = i / ig.n2;
             const int64_t j = r % ig.n1;
             const int64_t z = r / ig.n1;
-            const double w = quad_w(z, ig.n0, ig.dx0, ig.method) *
-                             quad_w(j, ig.n1, ig.dx1, ig.method) *
-                             quad_w(k, ig.n2, ig.dx2, ig.method);
-            add_sum(a.s, a.c, x[i] * (float)w);
+            const float w = (quad_k(z, ig.n0, ig.method) * quad_k(j, ig.n1, ig.method)) *
+                            quad_k(k, ig.n2, ig.method);
+            add_prod(a.s, a.c, x[i], w);
         }
     } else {
         for (int64_t i = t0; i < n; i += stride) {
@@ -162,8 +173,9 @@ __global__ void __launch_bounds__(kThreads) k_reduce_stage1(int kind, const floa
             rec[1] = r.hi;
             rec[2] = rec[3] = 0.0;
         } else {
-            rec[0] = r.s;
-            rec[1] = r.c;
+            const double sc = kind == SP_R_INTEGRATE ? ig.scale : 1.0;
+            rec[0] = (double)r.s * sc;
+            rec[1] = (double)r.c * sc;
             rec[2] = rec[3] = 0.0;
         }
     }
@@ -281,11 +293,11 @@ extern "C" int sp_reduce(int kind, const sp_tensor* x, double* result, void* wor
     SP_REQUIRE(result && workspace, "reduce: result and workspace are required");
     SP_REQUIRE(kind >= SP_R_SUM && kind <= SP_R_ANYSMALLDIAG, "reduce: unknown kind");
     int64_t n = numel(x);
-    IntegrateGeom ig{1, 1, 1, 1.0, 1.0, 1.0, 0};
+    IntegrateGeom ig{1, 1, 1, 1.0, 0};
     if (kind == SP_R_INTEGRATE) {
         SP_REQUIRE(x->ndim == 3 && wparams, "integrate: expected a 3-D field and weights");
-        ig = IntegrateGeom{x->shape[0], x->shape[1], x->shape[2], wparams[0], wparams[1],
-                           wparams[2], (int)wparams[3]};
+        ig = IntegrateGeom{x->shape[0], x->shape[1], x->shape[2],
+                           wparams[0] * wparams[1] * wparams[2] / 216.0, (int)wparams[3]};
     }
     if (kind == SP_R_ANYSMALLDIAG) {
         SP_REQUIRE(x->ndim >= 2 && x->shape[x->ndim - 1] == 3 && x->shape[x->ndim - 2] == 3,

@@ -1278,6 +1278,10 @@ class KernelGen:
             if op == "variance":
                 k1, k2 = uses[ri]
                 A(f"{ind}{{ RVar v_; v_.s1 = A.a{k1}; v_.s2 = A.a{k2}; block_var(v_, red_scratch, {rec}); }}")
+            elif op == "integrate":
+                dxs = _dx3(self.graph.node(nid).params.get("dx", 1.0))
+                scale = float(dxs[0]) * float(dxs[1]) * float(dxs[2]) / 216.0
+                A(f"{ind}block_sum(A.a{uses[ri][0]}, red_scratch, {rec}, {scale!r});")
             else:
                 fn = "block_mm" if op == "min_max" else "block_sum"
                 A(f"{ind}{fn}(A.a{uses[ri][0]}, red_scratch, {rec});")
@@ -1330,11 +1334,10 @@ class KernelGen:
             vals = e.val(src, pos)
             if kind == "int":
                 node = self.graph.node(self.group.reductions[ri])
-                dxs = _dx3(node.params.get("dx", 1.0))
                 meth = 0 if node.params.get("method", "simpson") == "simpson" else 1
-                w = (f"quad_weight(z, N0, {_flt(dxs[0])}, {meth}) * quad_weight(y, N1, {_flt(dxs[1])}, {meth})"
-                     f" * quad_weight(x, N2, {_flt(dxs[2])}, {meth})")
-                body.append(f"A.a{k}.add({vals[0]} * ({w}));")
+                # integer weight numerators (exact product); the FP64 record takes dx^3 / 216
+                w = f"(quad_k(z, N0, {meth}) * quad_k(y, N1, {meth})) * quad_k(x, N2, {meth})"
+                body.append(f"rsum_add_prod(A.a{k}, {vals[0]}, {w});")
             elif kind == "s2":
                 for v in vals:
                     body.append(f"rsum_add_sq(A.a{k}, {v});")
@@ -1902,8 +1905,10 @@ class KernelGen:
 
 
 # ------------------------------------------------------------------------------ compiled plans
+# FMA contraction across statements, except where a function asks for none (the compensated
+# accumulators: a product contracted into their running sum breaks the error terms)
 _RTC_OPTIONS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
-                "-Xclang", "-packed-fp32-ops", "-ffp-contract=fast"]
+                "-Xclang", "-packed-fp32-ops", "-ffp-contract=fast-honor-pragmas"]
 _cache_lock = threading.Lock()
 _module_cache: Dict[str, L.Module] = {}
 

@@ -280,3 +280,31 @@ def test_empty_field_point_wise():
     g, sink, *_ = sp.compile_expression("f**2 + tanh(f)*exp(f)", boundary="periodic", field_map={"f": f})
     out = sp.Runtime(device="cuda", fusion_enabled=True).run(g)[sink]
     assert tuple(out.shape) == (0, 4, 4)
+
+
+@pytest.mark.parametrize("shape", [(21, 24, 30), (9, 8, 131), (3, 5, 6)])
+def test_fused_quadrature_and_extrema(shape):
+    """integrate (Simpson and trapezoid, anisotropic spacing; odd and even axis lengths, where
+    Simpson's last interval differs), min_max and variance fused into the producing group.
+
+    The summed field is a periodic Laplacian with power-of-two spacings, whose FP32 values the
+    kernel reproduces bit for bit, so the reductions' errors are those of their own arithmetic.
+    (A sum over a field whose FP32 values are rounded differently, e.g. f*f + ... contracted
+    into an FMA, adds a random walk of per-voxel rounding differences that the rule, a
+    comparison with one other random walk, does not bound.)"""
+    f = field(shape, 31)
+    dx = (0.5, 1.0, 0.25)
+    g, sink, *_ = sp.compile_expression("laplacian(f)", dx=dx, boundary="periodic", field_map={"f": f})
+    reds = [g.add("integrate", (sink,), {"dx": dx, "method": "simpson"}),
+            g.add("integrate", (sink,), {"dx": dx, "method": "trapezoid"}),
+            g.add("integrate", (sink,), {}),
+            g.add("min_max", (sink,), {}), g.add("variance", (sink,), {})]
+    rt = sp.Runtime(device="cuda", fusion_enabled=True)
+    res = rt.fused.run(g, rt, wanted=[sink] + reds)
+    fused_reds = {r for _, _, comp in rt.fused._plans.values() for cg in comp.values()
+                  for r in cg.group.reductions}
+    assert fused_reds == set(reds)
+    o64 = graph_eval.evaluate(g, torch.float64)
+    o32 = graph_eval.evaluate(g, torch.float32)
+    for nid in [sink] + reds:
+        assert_2x(res[nid], o64[nid], o32[nid], f"{shape} {nid}")

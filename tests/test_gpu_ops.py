@@ -418,3 +418,35 @@ def test_statistics_in_graphs():
         o64 = graph_eval.evaluate(g, torch.float64)[sink]
         o32 = graph_eval.evaluate(g, torch.float32)[sink]
         assert_2x(out, o64, o32, expr)
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_hbm_probes_compute_their_mix(persistent):
+    """The bandwidth probes read and write what they claim: the 4:1 mixes sum each voxel's f and
+    its three u components (u planar as three thirds, or channel-last), the copy copies, the
+    write fills i-dependent values, the read sums per block.  Small integers, so every sum is
+    exact in FP32."""
+    from stenpy_amd import _lib as L
+    n = 4 * 3000
+    g = torch.Generator().manual_seed(5)
+    a = torch.randint(-50, 50, (n,), generator=g).float().cuda()
+    u = torch.randint(-50, 50, (n, 3), generator=g).float().cuda()
+    out = torch.empty_like(a)
+    L.probe_stream("4r1w_aos", persistent, a=a, u=u, out=out)
+    assert torch.equal(out, a + u.sum(1))
+    planar = u.t().contiguous().reshape(-1)          # three thirds: component c at c*n + i
+    L.probe_stream("4r1w", persistent, a=a, u=planar, out=out)
+    # the planar probe reads its thirds as if they were interleaved: voxel v's "components" are
+    # the floats 3v..3v+2 of the 4-voxel group's 12, drawn from the three thirds in turn
+    grp = torch.stack([planar[:n].view(-1, 4), planar[n:2 * n].view(-1, 4),
+                       planar[2 * n:].view(-1, 4)], 1).reshape(-1, 12).view(-1, 4, 3).sum(2).reshape(-1)
+    assert torch.equal(out, a + grp)
+    L.probe_stream("copy", persistent, a=a, out=out)
+    assert torch.equal(out, a)
+    L.probe_stream("write", persistent, a=out)
+    i = torch.arange(n // 4, device="cuda")
+    want = ((i & 1023).float().unsqueeze(1) + torch.arange(4, device="cuda").float()).reshape(-1)
+    assert torch.equal(out, want)
+    part = torch.empty(L.probe_blocks(persistent, n), dtype=torch.float64, device="cuda")
+    L.probe_stream("read", persistent, a=a, partials=part)
+    assert float(part.sum()) == float(a.double().sum())
