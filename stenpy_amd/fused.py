@@ -552,11 +552,13 @@ class _Emit:
             slot = "sq"
         else:
             slot = f"((unsigned)(s{kz:+d}) % {src['D']}u)" if kz else f"((unsigned)s % {src['D']}u)"
-        H = src["H"]
         rows = []
         for a, (lvar, t0, T, N) in ((1, ("ly", "y0", "TY", "N1")), (2, ("lx", "x0", "TX", "N2"))):
+            H = src["HY"] if a == 1 else src["H"]
             c = pos[a]
             if c[0] == "I":
+                if abs(c[1]) > H:
+                    raise RuntimeError(f"LDS tap {c[1]:+d} on axis {a} outside the halo of {H}")
                 rows.append(f"({lvar}+{H + c[1]})")
             elif c[0] == "A":
                 # the x-face tile starts at 0 (low face) or N2 - TX (high face); same mapping
@@ -977,7 +979,7 @@ class KernelGen:
             raise ValueError(f"schedule {sched} does not map threads onto whole tile rows")
         self.rp = sched.threads // sched.tx          # rows per pass
         self.v = sched.ty // self.rp                 # voxels per thread per plane
-        self.src_layout = [{"C": info[s].ch, "H": reach[s][2], "D": 2 * self.hz + 2,
+        self.src_layout = [{"C": info[s].ch, "H": reach[s][2], "HY": reach[s][2], "D": 2 * self.hz + 2,
                             "mode": "ring", "lds": True} for s in group.sources]
         self.xform = self._load_transforms()
         self.xform_node = {v[1]: si for si, v in self.xform.items()}
@@ -991,6 +993,12 @@ class KernelGen:
             edge_mode = "band"
         self.edge_mode = self._pick_edge_mode(edge_mode or EDGE_MODE)
         self._choose_modes()
+        # In band mode every LDS evaluation of an in-plane tap sits at an interior row (the y
+        # faces go to the gather blocks), so the rows above and below the tile need only the
+        # interior reach; the columns keep the full halo, which the face wave's wrapped
+        # one-sided x formulas read (a periodic op over a one-sided one widens it).
+        for s_, L_ in zip(group.sources, self.src_layout):
+            L_["HY"] = min(L_["H"], reach[s_][0]) if self.edge_mode == "band" else L_["H"]
         # outputs of >= 4 channels written through a per-wave LDS transpose (coalesced lines)
         self.stage = [oi for oi, o in enumerate(group.outputs)
                       if STAGE_STORES and self.use_lds and not batched and (sched.tx % 64 == 0 or 64 % sched.tx == 0)
@@ -1117,7 +1125,7 @@ class KernelGen:
             if self.hz == 1 and zoff:
                 L_.update(mode="queue", D=2, lds=inplane)
                 if not inplane:
-                    L_["H"] = 0
+                    L_["H"] = L_["HY"] = 0
         # the edge variants may still need a ring (e.g. a wrapped coordinate off the column)
         while True:
             try:
@@ -1414,7 +1422,7 @@ class KernelGen:
 
     def lds_array_bytes(self) -> int:
         s = self.sched
-        return sum(L_["D"] * L_["C"] * (s.ty + 2 * L_["H"] + 1) * (s.tx + 2 * L_["H"]) * 4
+        return sum(L_["D"] * L_["C"] * (s.ty + 2 * L_["HY"] + 1) * (s.tx + 2 * L_["H"]) * 4
                    for L_ in (self.src_layout[i] for i in self.lds_sources()))
 
     def _body(self, idx64: bool = False) -> List[str]:
@@ -1469,10 +1477,10 @@ class KernelGen:
         off = 0
         for i in self.lds_sources():
             L_ = self.src_layout[i]
-            dims = f"[{L_['D']}][{L_['C']}][TY + {2 * L_['H'] + 1}][TX + {2 * L_['H']}]"
+            dims = f"[{L_['D']}][{L_['C']}][TY + {2 * L_['HY'] + 1}][TX + {2 * L_['H']}]"
             if self.batched:
                 lines.append(f"#define lds_s{i} (*reinterpret_cast<float (*){dims}>(sp_smem + {off}))")
-                off += L_["D"] * L_["C"] * (s.ty + 2 * L_["H"] + 1) * (s.tx + 2 * L_["H"])
+                off += L_["D"] * L_["C"] * (s.ty + 2 * L_["HY"] + 1) * (s.tx + 2 * L_["H"])
             else:
                 # one spare row per channel (the last channel's absorbs the stores of unused
                 # halo entries)
@@ -1650,15 +1658,15 @@ class KernelGen:
         L_ = self.src_layout[i]
         if not L_["lds"]:
             return 0
-        H, C = L_["H"], L_["C"]
+        H, HY, C = L_["H"], L_["HY"], L_["C"]
         W = self.sched.tx + 2 * H
-        return (2 * H * W + 2 * H * self.sched.ty) * C
+        return (2 * HY * W + 2 * H * self.sched.ty) * C
 
     def _emit_loaders(self, A) -> None:
         s = self.sched
         for i, L_ in enumerate(self.src_layout):
-            C, H = L_["C"], L_["H"]
-            W, RH = s.tx + 2 * H, s.ty + 2 * H
+            C, H, HY = L_["C"], L_["H"], L_["HY"]
+            W, RH = s.tx + 2 * H, s.ty + 2 * HY
             HT = self._halo_count(i)
             KH = (HT + s.threads - 1) // s.threads
             L_["KH"] = KH
@@ -1676,15 +1684,15 @@ class KernelGen:
                 A(f"    for (int j = 0; j < {KH}; ++j) {{")
                 A(f"        const int e = tid < NT ? tid + j * NT : {HT};")
                 A(f"        int r, col, ch;")
-                top = H * W * C
+                top = HY * W * C
                 mid = s.ty * 2 * H * C
                 A(f"        if (e < {top}) {{ r = e / {W * C}; const int q = e - r * {W * C}; col = q / {C}; ch = q - col * {C}; }}")
-                A(f"        else if (e < {top + mid}) {{ const int e2 = e - {top}; r = {H} + e2 / {2 * H * C};"
-                  f" const int q = e2 - (r - {H}) * {2 * H * C}; const int jj = q / {C}; ch = q - jj * {C};"
+                A(f"        else if (e < {top + mid}) {{ const int e2 = e - {top}; r = {HY} + e2 / {2 * H * C};"
+                  f" const int q = e2 - (r - {HY}) * {2 * H * C}; const int jj = q / {C}; ch = q - jj * {C};"
                   f" col = jj < {H} ? jj : TX + jj; }}")
-                A(f"        else {{ const int e3 = e - {top + mid}; r = TY + {H} + e3 / {W * C};"
-                  f" const int q = e3 - (r - TY - {H}) * {W * C}; col = q / {C}; ch = q - col * {C}; }}")
-                A(f"        hg{i}[j] = (wrapm(y0 - {H} + r, N1) * N2 + wrapm(x0 - {H} + col, N2)) * {C} + ch;")
+                A(f"        else {{ const int e3 = e - {top + mid}; r = TY + {HY} + e3 / {W * C};"
+                  f" const int q = e3 - (r - TY - {HY}) * {W * C}; col = q / {C}; ch = q - col * {C}; }}")
+                A(f"        hg{i}[j] = (wrapm(y0 - {HY} + r, N1) * N2 + wrapm(x0 - {H} + col, N2)) * {C} + ch;")
                 # out-of-range entries write a spare word at the end of the slot array
                 A(f"        hl{i}[j] = e < {HT} ? (ch * {RH + 1} + r) * {W} + col : {((C - 1) * (RH + 1) + RH) * W};")
                 A("    }")
@@ -1740,13 +1748,13 @@ class KernelGen:
             self._xf_own(A, i, buf, indent, own)
         if not L_["lds"]:
             return
-        C, H, KH = L_["C"], L_["H"], L_.get("KH", 0)
-        W, RH = self.sched.tx + 2 * H, self.sched.ty + 2 * H
+        C, H, HY, KH = L_["C"], L_["H"], L_["HY"], L_.get("KH", 0)
+        W, RH = self.sched.tx + 2 * H, self.sched.ty + 2 * HY
         A(f"{indent}{{ float* ls = &lds_s{i}[{slot}][0][0][0];")
         A(f"{indent}  #pragma unroll")
         A(f"{indent}  for (int k = 0; k < VPT; ++k)")
         A(f"{indent}    #pragma unroll")
-        A(f"{indent}    for (int c = 0; c < {C}; ++c) ls[(c * {RH + 1} + ly0 + k * RP + {H}) * {W} + lx + {H}] = {own}[k][c];")
+        A(f"{indent}    for (int c = 0; c < {C}; ++c) ls[(c * {RH + 1} + ly0 + k * RP + {HY}) * {W} + lx + {H}] = {own}[k][c];")
         if KH:
             xf = (lambda v: f"xf{i}({v})") if i in self.xform else (lambda v: v)
             A(f"{indent}  #pragma unroll")

@@ -42,6 +42,11 @@ def test_reduction_result_starts_a_new_level():
 def test_periodic_over_one_sided_widens_the_halo():
     *_, comp = plan("mean_curvature(f)", bc="periodic")
     assert comp[0].gen.src_layout[0]["H"] == 3
+    # band mode (a grid where the band is a small share): rows above and below the tile need
+    # only the interior reach (the face wave's wrapped one-sided formulas run along x)
+    *_, comp = plan("mean_curvature(f)", bc="periodic", shape=(64, 96, 128))
+    assert comp[0].gen.edge_mode == "band"
+    assert comp[0].gen.src_layout[0]["H"] == 3 and comp[0].gen.src_layout[0]["HY"] == 2
     *_, comp = plan("mean_curvature(f)", bc="neumann")
     assert comp[0].gen.src_layout[0]["H"] == 2
 
