@@ -206,6 +206,14 @@ def bench_batched(args) -> int:
     seq = lambda: [rt.fused.run(g, rt, wanted=w) for g, w in items]
     t_seq = time_steps(seq, max(2, args.steps // 5), 1)
     vox = st["voxels"]
+    # HBM bytes per step from the committed FETCH_SIZE / WRITE_SIZE passes of this workload, when
+    # they were taken on the same batch (same algorithmic bytes)
+    traffic = None
+    tp = ROOT / "profiles" / "traffic_batched_96.json"
+    if tp.exists():
+        tj = json.loads(tp.read_text())
+        if tj.get("algorithmic_bytes_per_step") == st["algorithmic_bytes"]:
+            traffic = round(tj["bytes_per_step"])
     _line("batched persistent launch: Gvoxel/s over 96 uneven grids x 6 random graphs",
           round(vox / t / 1e9, 2), "Gvoxel/s", args.steps, args.warmup, t * 1e3,
           {"workload": "96 fields, axes U{32..384} seed 6; 6 random graphs (3-12 ops) seed 6; periodic",
@@ -217,7 +225,7 @@ def bench_batched(args) -> int:
           roofline={"bound": "hbm", "achieved": round(st["algorithmic_bytes"] / t / 1e9, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(st["algorithmic_bytes"] / t / 1e9 / HBM_PEAK_GBS, 4),
-                    "traffic": None, "algorithmic_bytes_per_launch": st["algorithmic_bytes"]})
+                    "traffic": traffic, "algorithmic_bytes_per_launch": st["algorithmic_bytes"]})
     return 0
 
 
