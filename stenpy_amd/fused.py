@@ -99,6 +99,19 @@ def heuristic_schedule(grid: Sequence[int]) -> Schedule:
     while zc > 32 and (zc >= n0 or tiles * -(-n0 // zc) < 512):
         zc //= 2
     return Schedule(tx, ty, zc, nt)
+def fill_small_grid(s: Schedule, grid: Sequence[int], min_blocks: int = 512, min_zc: int = 8) -> Schedule:
+    """``s`` with its z-chunk halved until the launch has ``min_blocks`` blocks (two per CU) or
+    the chunk reaches ``min_zc`` planes.  The schedule search and the model rank chunks of 32
+    planes and more, which leave a small grid (128^3: 256 blocks of a 64 x 8 tile) to a
+    fraction of the GPU's 256 CUs."""
+    n0, n1, n2 = (int(v) for v in grid)
+    tiles = -(-n2 // s.tx) * -(-n1 // s.ty)
+    zc = s.zc
+    while zc > min_zc and tiles * -(-n0 // zc) < min_blocks:
+        zc //= 2
+    return s if zc == s.zc else Schedule(s.tx, s.ty, max(zc, min_zc), s.threads)
+
+
 # LDS per block kept within 80 KiB so that at least two blocks share a CU (160 KiB per CU)
 LDS_LIMIT = 80 * 1024
 FALLBACK_SCHEDULES = [Schedule(64, 4, 64, 256), Schedule(32, 8, 64, 256), Schedule(32, 4, 64, 128),
@@ -1136,11 +1149,22 @@ class KernelGen:
                 L_ = self.src_layout[nr.si]
                 L_.update(mode="ring", D=2 * self.hz + 2, lds=True)
 
+    def _memo(self, name: str, fn):
+        """Value of a geometry query on the generator's own grid, computed once: the grid,
+        schedule, edge mode and band are fixed at construction, and a run asks several times."""
+        memo = self.__dict__.setdefault("_geom_memo", {})
+        if name not in memo:
+            memo[name] = fn()
+        return memo[name]
+
     def xface_sides(self) -> List[str]:
         """x faces whose band the main kernel evaluates with compile-time face variants.
 
         A face voxel sits in its own cache line per row and plane, so leaving the x faces to
         the gather path re-reads whole lines long after the LDS kernel fetched them."""
+        return self._memo("xface_sides", self._xface_sides)
+
+    def _xface_sides(self) -> List[str]:
         if not self.use_lds or self.edge_mode != "band" or self.redge == 0 or self.batched:
             return []
         n2, tx, r = self.grid[2], self.sched.tx, self.redge
@@ -1175,6 +1199,11 @@ class KernelGen:
     # -- boundary kernel -------------------------------------------------------------------------
     def boundary_boxes(self, grid=None) -> List[Tuple[int, int, int, int, int, int]]:
         """Disjoint boxes (z0, z1, y0, y1, x0, x1) the main kernel leaves out."""
+        if grid is None:
+            return self._memo("boundary_boxes", lambda: self._boundary_boxes(None))
+        return self._boundary_boxes(grid)
+
+    def _boundary_boxes(self, grid) -> List[Tuple[int, int, int, int, int, int]]:
         if not self.use_lds:
             return []
         n0, n1, n2 = grid or self.grid
@@ -1191,6 +1220,11 @@ class KernelGen:
     TAIL_VPT = 4     # voxels per thread of a tail block
 
     def boundary_blocks(self, grid=None) -> int:
+        if grid is None:
+            return self._memo("boundary_blocks", lambda: self._boundary_blocks(None))
+        return self._boundary_blocks(grid)
+
+    def _boundary_blocks(self, grid) -> int:
         tot = sum((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]) for b in self.boundary_boxes(grid))
         per = self.block_threads() * self.TAIL_VPT
         return (tot + per - 1) // per
@@ -2121,7 +2155,7 @@ class FusedExecutor:
             s = self.schedule_fn(group, graph)
             if s is not None:
                 return s
-        return heuristic_schedule(group.grid)
+        return fill_small_grid(heuristic_schedule(group.grid), group.grid)
 
     @staticmethod
     def _fit_registers(gen: "KernelGen", src: str) -> str:

@@ -136,7 +136,8 @@ def schedule_fn(device: Optional[torch.device] = None, use_model: bool = True,
         if use_model:
             m = load_default_model(device or "cuda")
             if m is not None:
-                return m.best(group, graph, threads)
+                s = m.best(group, graph, threads)
+                return F.fill_small_grid(s, group.grid) if s is not None else None
         return None
 
     return pick

@@ -103,7 +103,7 @@ def test_hessian_tensor_ops(edge_mode):
 
 @pytest.mark.parametrize("edge_mode", ["band", "variants"])
 @pytest.mark.parametrize("sched", [Schedule(128, 8, 16, 256), Schedule(64, 4, 16, 256), Schedule(32, 8, 16, 128),
-                                   Schedule(16, 8, 16, 128)])
+                                   Schedule(16, 8, 16, 128), Schedule(32, 8, 8, 128)])
 def test_wide_outputs_staged_stores(sched, edge_mode):
     # 9-channel outputs go through the per-wave LDS transpose (a wave spans 64 / TX rows of
     # narrow tiles); x = 200 and y = 20 leave partial tiles whose lanes past the edge store nothing

@@ -70,3 +70,17 @@ def test_model_shortlist_respects_a_block_size_filter(monkeypatch):
     m = tune.ScheduleModel(_Mlp())
     assert m.best(grp, g).threads == 512
     assert m.best(grp, g, threads=256).threads == 256
+
+
+def test_fill_small_grid_halves_the_chunk_until_the_gpu_is_filled():
+    from stenpy_amd.fused import Schedule, fill_small_grid
+    # 128^3 with a 64 x 8 tile: 32 tiles per plane, 4 chunks of 32 planes -> 128 blocks
+    s = fill_small_grid(Schedule(64, 8, 32, 256), (128, 128, 128))
+    assert s == Schedule(64, 8, 8, 256)          # 16 chunks of 8 planes: 512 blocks
+    # a large grid keeps its schedule (512 blocks of 64 x 16 x 256 at 512^3)
+    big = Schedule(64, 16, 256, 512)
+    assert fill_small_grid(big, (512, 512, 512)) is big
+    # halving stops as soon as two blocks per CU are reached
+    assert fill_small_grid(Schedule(32, 8, 64, 128), (128, 128, 128)) == Schedule(32, 8, 16, 128)
+    # and at the 8-plane floor, whatever the block count
+    assert fill_small_grid(Schedule(64, 16, 64, 512), (32, 32, 64)) == Schedule(64, 16, 8, 512)
