@@ -55,11 +55,14 @@ def splitmix64_u53(seed: int, lin: np.ndarray) -> np.ndarray:
     return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
 
-def generate_host(shape: Sequence[int], seed: int, z0: int = 0, nz: int = None) -> np.ndarray:
-    """FP32 planes ``[z0, z0+nz)`` of the field of full ``shape`` (n0, n1, n2), on the host."""
+def generate_host(shape: Sequence[int], seed: int, z0: int = 0, nz: int = None,
+                  modes=None) -> np.ndarray:
+    """FP32 planes ``[z0, z0+nz)`` of the field of full ``shape`` (n0, n1, n2), on the host,
+    by the direct formula (one cosine per voxel and mode).  ``modes`` = (ks, amps, phis)
+    replaces the seed's modes (the noise still follows ``seed``)."""
     n0, n1, n2 = (int(s) for s in shape)
     nz = n0 - z0 if nz is None else int(nz)
-    ks, amps, phis = fourier_params(seed)
+    ks, amps, phis = fourier_params(seed) if modes is None else modes
     i = np.arange(z0, z0 + nz, dtype=np.float64)[:, None, None]
     j = np.arange(n1, dtype=np.float64)[None, :, None]
     k = np.arange(n2, dtype=np.float64)[None, None, :]

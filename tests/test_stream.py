@@ -130,3 +130,20 @@ def test_producer_error_propagates():
     g, sink, _, fm, _ = sp.compile_expression(STREAM_EXPR, boundary="periodic", field_map={"f": probe})
     with pytest.raises(RuntimeError, match="disk on fire"):
         stream_graph(g, sink, fm["f"], lf, slab_planes=8)
+
+
+@pytest.mark.parametrize("distinct", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_host_generator_merges_modes_sharing_an_x_wavenumber(distinct):
+    """Modes with the same x wavenumber share one term per voxel in the slab generator; every
+    number of distinct wavenumbers (1..8, one code path each) gives the direct formula's
+    values (cos per voxel and mode, numpy), to one ulp at most and almost always exactly."""
+    from stenpy_amd import _lib as L
+    ks, amps, phis = gen.fourier_params(5)
+    ks = ks.copy()
+    ks[:, 2] = [1 + (m % distinct) for m in range(8)]
+    shape, z0, nz = (9, 7, 300), 2, 5
+    want = gen.generate_host(shape, 5, z0=z0, nz=nz, modes=(ks, amps, phis))
+    got = np.empty((nz,) + shape[1:], np.float32)
+    L.host_gen_fourier(got.ctypes.data, shape, z0, nz, ks, amps, phis, 5, 2)
+    np.testing.assert_array_max_ulp(got, want, maxulp=1)
+    assert np.count_nonzero(got != want) <= 2
