@@ -271,7 +271,7 @@ def bench_tensor(args) -> int:
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel_ms": round(k_ms, 4), "algorithmic_bytes_per_launch": alg,
                     "kernels": "fused_flag (+ fused_flag_tail), fused_kernel, fused_tail (side stream)",
-                    "note": "VALU issue is the co-limit: counters give ~63% of the kernel's cycles "
+                    "note": "VALU issue is the co-limit: counters give ~64% of the kernel's cycles "
                             "issuing VALU (profiles/tensor_768_fused_kernel_counters.md)"},
           parity=parity)
     return 0
